@@ -1,0 +1,649 @@
+// C-ABI of the MI355X (T) hot path: contexts, resident systems, the tuple
+// window scheduler and the run loop.  See include/atrip_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace atrip_amd;
+
+namespace {
+
+struct Fail {
+  std::string msg;
+};
+
+void check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw Fail{std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+template <typename F>
+int guarded(atrip_hip_ctx *ctx, F &&f) {
+  if (!ctx) return 1;
+  try {
+    ctx->err.clear();
+    f();
+    return 0;
+  } catch (const Fail &x) {
+    ctx->err = x.msg;
+  } catch (const std::exception &x) {
+    ctx->err = x.what();
+  }
+  if (ctx->err.empty()) ctx->err = "unknown failure";
+  return 2;
+}
+
+double *dev_alloc(DevSystem &s, int64_t n, const char *what) {
+  void *p = nullptr;
+  const size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(double);
+  check(hipMalloc(&p, bytes), what);
+  s.allocations.push_back(p);
+  s.bytes += bytes;
+  check(hipMemset(p, 0, bytes), what);
+  return (double *)p;
+}
+
+void free_system(DevSystem &s) {
+  for (void *p : s.allocations) (void)hipFree(p);
+  s.allocations.clear();
+}
+
+uint64_t n_tuples(uint64_t Nv) { return Nv * (Nv + 1) * (Nv + 2) / 6 - Nv; }
+
+void alloc_system(atrip_hip_ctx *ctx, DevSystem &s, int64_t No, int64_t Nv,
+                  bool with_j) {
+  if (No < 1 || Nv < 1) throw Fail{"No and Nv must be positive"};
+  if (No > 1023) throw Fail{"No > 1023 is not supported"};
+  s.No = No;
+  s.Nv = Nv;
+  s.Kv = pad16(Nv);
+  s.Ko = pad16(No);
+  const int64_t N2 = No * No, Nv2 = Nv * Nv;
+  s.T3 = dev_alloc(s, Nv * N2 * s.Kv, "alloc T3");
+  s.V3 = dev_alloc(s, Nv2 * No * s.Kv, "alloc V3");
+  s.H3 = dev_alloc(s, Nv * N2 * s.Ko, "alloc H3");
+  s.T2N = dev_alloc(s, Nv2 * No * s.Ko, "alloc T2N");
+  s.T2TN = dev_alloc(s, Nv2 * No * s.Ko, "alloc T2TN");
+  s.VHH = dev_alloc(s, Nv2 * N2, "alloc VHH");
+  s.Tai = dev_alloc(s, Nv * No, "alloc Tai");
+  s.ei = dev_alloc(s, No, "alloc ei");
+  s.ea = dev_alloc(s, Nv, "alloc ea");
+  if (with_j) {
+    s.V3J = dev_alloc(s, Nv2 * No * s.Kv, "alloc V3J");
+    s.H3J = dev_alloc(s, Nv * N2 * s.Ko, "alloc H3J");
+  }
+  // i >= j >= k points of the energy loops, packed 10 bits per index
+  std::vector<uint32_t> pts;
+  for (int64_t k = 0; k < No; k++)
+    for (int64_t j = k; j < No; j++)
+      for (int64_t i = j; i < No; i++)
+        pts.push_back((uint32_t)(i | (j << 10) | (k << 20)));
+  s.npoints = (int64_t)pts.size();
+  {
+    void *p = nullptr;
+    check(hipMalloc(&p, pts.size() * sizeof(uint32_t)), "alloc points");
+    s.allocations.push_back(p);
+    check(hipMemcpy(p, pts.data(), pts.size() * sizeof(uint32_t),
+                    hipMemcpyHostToDevice),
+          "upload points");
+    s.points = (uint32_t *)p;
+  }
+  // tuples with first index < a, for the canonical index
+  std::vector<uint64_t> pre(Nv + 1, 0);
+  for (int64_t a = 0; a < Nv; a++) {
+    const uint64_t m = (uint64_t)(Nv - a);
+    pre[a + 1] = pre[a] + m * (m + 1) / 2 - 1;
+  }
+  {
+    void *p = nullptr;
+    check(hipMalloc(&p, pre.size() * sizeof(uint64_t)), "alloc prefix");
+    s.allocations.push_back(p);
+    check(hipMemcpy(p, pre.data(), pre.size() * sizeof(uint64_t),
+                    hipMemcpyHostToDevice),
+          "upload prefix");
+    s.aprefix = (uint64_t *)p;
+  }
+  (void)ctx;
+}
+
+// Stage a host tensor through a bounded device buffer, `slab` elements per
+// step of its slowest CTF index (n_slow steps), and hand each staged chunk
+// to `place(dev_chunk, first_slow, count_slow)`.
+template <typename P>
+void staged(atrip_hip_ctx *ctx, const double *host, int64_t slab,
+            int64_t n_slow, P &&place) {
+  const int64_t budget = (int64_t)1 << 29;  // 4 GiB of doubles
+  int64_t per = std::max<int64_t>(1, budget / std::max<int64_t>(slab, 1));
+  per = std::min(per, n_slow);
+  void *stage = nullptr;
+  check(hipMalloc(&stage, (size_t)(per * slab) * sizeof(double)),
+        "alloc staging");
+  try {
+    for (int64_t s0 = 0; s0 < n_slow; s0 += per) {
+      const int64_t cnt = std::min(per, n_slow - s0);
+      check(hipMemcpyAsync(stage, host + s0 * slab,
+                           (size_t)(cnt * slab) * sizeof(double),
+                           hipMemcpyHostToDevice, ctx->stream),
+            "upload chunk");
+      place((const double *)stage, s0, cnt);
+      check(hipStreamSynchronize(ctx->stream), "relayout");
+    }
+  } catch (...) {
+    (void)hipFree(stage);
+    throw;
+  }
+  check(hipFree(stage), "free staging");
+}
+
+void upload_host(atrip_hip_ctx *ctx, DevSystem &s, const atrip_sys_desc *d) {
+  const int64_t No = s.No, Nv = s.Nv, N2 = No * No, Nv2 = Nv * Nv;
+  const int64_t Kv = s.Kv, Ko = s.Ko;
+  hipStream_t st = ctx->stream;
+  // Tabij {Nv,Nv,No,No}: slowest index j
+  staged(ctx, d->Tabij, Nv2 * No, No,
+         [&](const double *c, int64_t j0, int64_t cj) {
+           // T3[x][p + No q][E] = T[x,E,p,q], rows of this chunk: q = j
+           check(launch_transpose(c, s.T3 + j0 * No * Kv, No * cj, Nv, Nv,
+                                  Nv2, Nv, 1, Kv, 1, N2 * Kv, 1.0, st),
+                 "relayout T3");
+           // T2N[pair][n][L] = -T[pair,L,n], n = j
+           check(launch_transpose(c, s.T2N + j0 * Ko, cj, No, Nv2, Nv2 * No,
+                                  Nv2, 1, Ko, 1, No * Ko, -1.0, st),
+                 "relayout T2N");
+           // T2TN[pair][n][L] = -T[pair,n,L], L = j
+           check(launch_transpose(c, s.T2TN + j0, cj, No, Nv2, Nv2 * No, Nv2,
+                                  1, 1, Ko, No * Ko, -1.0, st),
+                 "relayout T2TN");
+         });
+  // Vabij {Nv,Nv,No,No}: VHH[pair][i + No j]
+  staged(ctx, d->Vabij, Nv2 * No, No,
+         [&](const double *c, int64_t j0, int64_t cj) {
+           check(launch_transpose(c, s.VHH + j0 * No, 1, No * cj, Nv2, 0, Nv2,
+                                  1, 0, 1, N2, 1.0, st),
+                 "relayout VHH");
+         });
+  // Vijka {No,No,No,Nv}: H3[x][row][L] = V[row,L,x]
+  auto hhh = [&](const double *host, double *dst) {
+    staged(ctx, host, N2 * No, Nv,
+           [&](const double *c, int64_t x0, int64_t cx) {
+             check(launch_transpose(c, dst + x0 * N2 * Ko, cx, No, N2,
+                                    N2 * No, N2, 1, N2 * Ko, 1, Ko, 1.0, st),
+                   "relayout H3");
+           });
+  };
+  // Vabci {Nv,Nv,Nv,No}: V3[pair][r][E] = V[pair,E,r]
+  auto ppph = [&](const double *host, double *dst) {
+    staged(ctx, host, Nv2 * Nv, No,
+           [&](const double *c, int64_t r0, int64_t cr) {
+             check(launch_transpose(c, dst + r0 * Kv, cr, Nv, Nv2, Nv2 * Nv,
+                                    Nv2, 1, Kv, 1, No * Kv, 1.0, st),
+                   "relayout V3");
+           });
+  };
+  hhh(d->Vijka, s.H3);
+  ppph(d->Vabci, s.V3);
+  if (s.V3J) {
+    hhh(d->Jijka, s.H3J);
+    ppph(d->Jabci, s.V3J);
+  }
+  check(hipMemcpy(s.Tai, d->Tai, sizeof(double) * Nv * No,
+                  hipMemcpyHostToDevice),
+        "upload Tai");
+  check(hipMemcpy(s.ei, d->ei, sizeof(double) * No, hipMemcpyHostToDevice),
+        "upload ei");
+  check(hipMemcpy(s.ea, d->ea, sizeof(double) * Nv, hipMemcpyHostToDevice),
+        "upload ea");
+}
+
+DevSystem &get_sys(atrip_hip_ctx *ctx, int sys) {
+  auto it = ctx->systems.find(sys);
+  if (it == ctx->systems.end()) throw Fail{"unknown system handle"};
+  return it->second;
+}
+
+// Window plan for a system at the context's window edge.
+WindowPlan &get_plan(atrip_hip_ctx *ctx, int sys, const DevSystem &s) {
+  auto it = ctx->plans.find(sys);
+  if (it != ctx->plans.end() && it->second.nb == ctx->window) return it->second;
+  if (it != ctx->plans.end()) {
+    (void)hipFree(it->second.d_windows);
+    ctx->plans.erase(it);
+  }
+  WindowPlan p;
+  p.nb = ctx->window;
+  p.nblk = (int)((s.Nv + p.nb - 1) / p.nb);
+  for (int al = 0; al < p.nblk; al++)
+    for (int be = al; be < p.nblk; be++)
+      for (int ga = be; ga < p.nblk; ga++) {
+        p.windows.push_back(al);
+        p.windows.push_back(be);
+        p.windows.push_back(ga);
+      }
+  return ctx->plans.emplace(sys, std::move(p)).first->second;
+}
+
+void ensure(void *&ptr, size_t &have, size_t need, const char *what) {
+  if (have >= need && ptr) return;
+  if (ptr) check(hipFree(ptr), what);
+  ptr = nullptr;
+  have = 0;
+  check(hipMalloc(&ptr, need), what);
+  have = need;
+}
+
+// Host-side canonical index, identical to the device one.
+struct HostIndex {
+  std::vector<uint64_t> pre;
+  uint64_t Nv;
+  explicit HostIndex(uint64_t nv) : pre(nv + 1, 0), Nv(nv) {
+    for (uint64_t a = 0; a < nv; a++) {
+      const uint64_t m = nv - a;
+      pre[a + 1] = pre[a] + m * (m + 1) / 2 - 1;
+    }
+  }
+  uint64_t operator()(uint64_t a, uint64_t b, uint64_t c) const {
+    return tuple_index(pre.data(), Nv, a, b, c);
+  }
+};
+
+struct RunResult {
+  double e = 0, ect = 0;
+};
+
+// Runs every batch of windows intersecting [first, last).  When `sink` is
+// given it receives the per-slot energies of each batch.
+template <typename Sink>
+RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
+                    uint64_t first, uint64_t last, Sink &&sink) {
+  DevSystem &s = get_sys(ctx, sys);
+  WindowPlan &plan = get_plan(ctx, sys, s);
+  const uint64_t n = n_tuples(s.Nv);
+  last = std::min(last, n);
+  RunResult res;
+  if (first >= last) return res;
+  const int nb = plan.nb;
+  const bool full = first == 0 && last == n;
+  HostIndex hidx((uint64_t)s.Nv);
+  // select windows and count their tuples in range
+  std::vector<int32_t> sel;
+  std::vector<uint64_t> counts;
+  const size_t nwin = plan.windows.size() / 3;
+  for (size_t w = 0; w < nwin; w++) {
+    const int64_t al = plan.windows[3 * w], be = plan.windows[3 * w + 1],
+                  ga = plan.windows[3 * w + 2];
+    uint64_t cnt = 0;
+    for (int ta = 0; ta < nb; ta++)
+      for (int tb = 0; tb < nb; tb++)
+        for (int tc = 0; tc < nb; tc++) {
+          const uint64_t a = al * nb + ta, b = be * nb + tb, c = ga * nb + tc;
+          if (!tuple_ok(s.Nv, a, b, c)) continue;
+          if (!full) {
+            const uint64_t q = hidx(a, b, c);
+            if (q < first || q >= last) continue;
+          }
+          cnt++;
+        }
+    if (!cnt) continue;
+    sel.insert(sel.end(), {(int32_t)al, (int32_t)be, (int32_t)ga});
+    counts.push_back(cnt);
+  }
+  const int nsel = (int)counts.size();
+  if (!nsel) return res;
+  void *d_sel = nullptr;
+  check(hipMalloc(&d_sel, sel.size() * sizeof(int32_t)), "alloc windows");
+  struct FreeOnExit {
+    void *p;
+    ~FreeOnExit() { (void)hipFree(p); }
+  } guard{d_sel};
+  check(hipMemcpy(d_sel, sel.data(), sel.size() * sizeof(int32_t),
+                  hipMemcpyHostToDevice),
+        "upload windows");
+
+  const int bw = std::max(1, ctx->batch);
+  const int64_t nb3 = (int64_t)nb * nb * nb;
+  const int64_t slots_max = (int64_t)std::min(bw, nsel) * nb3;
+  const int64_t N3 = s.No * s.No * s.No;
+  const bool use_j = s.V3J != nullptr;
+  const size_t gsz = (size_t)(3 * slots_max * N3) * sizeof(double);
+  ensure((void *&)ctx->G, ctx->G_bytes, use_j ? 2 * gsz : gsz, "alloc G");
+  ensure((void *&)ctx->e_slot, ctx->e_slot_bytes,
+         (size_t)(2 * slots_max) * sizeof(double), "alloc e_slot");
+  const int nbatch = (nsel + bw - 1) / bw;
+  ensure((void *&)ctx->partials, ctx->partials_bytes,
+         (size_t)(4 * nbatch) * sizeof(double), "alloc partials");
+  double *G = ctx->G;
+  double *GJ = use_j ? ctx->G + 3 * slots_max * N3 : nullptr;
+  const double flop_tuple = 12.0 * (double)s.No * s.No * s.No * (s.No + s.Nv);
+  const char *dname = precision == ATRIP_FP64 ? "doubles" : "doubles_mixed";
+  std::vector<double> host_slots;
+
+  for (int bi = 0; bi < nbatch; bi++) {
+    BatchArgs b;
+    const int w0 = bi * bw;
+    b.nw = std::min(bw, nsel - w0);
+    b.windows = (const int32_t *)d_sel + 3 * w0;
+    b.nb = nb;
+    b.first = full ? 0 : first;
+    b.last = full ? ~0ULL : last;
+    b.slots = (int64_t)b.nw * nb3;
+    uint64_t cnt = 0;
+    for (int w = 0; w < b.nw; w++) cnt += counts[w0 + w];
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    if (ctx->timing) {
+      check(hipEventCreate(&e0), "event");
+      check(hipEventCreate(&e1), "event");
+      check(hipEventCreate(&e2), "event");
+      check(hipEventRecord(e0, ctx->stream), "event");
+    }
+    for (int pass = 0; pass < (use_j ? 2 : 1); pass++) {
+      double *Gp = pass == 0 ? G : GJ;
+      if (precision == ATRIP_FP64)
+        check(launch_doubles_f64(s, b, pass == 1, Gp, ctx->tile, ctx->stream),
+              "doubles kernel");
+      else if (precision == ATRIP_SPLIT_FP32)
+        check(launch_doubles_split(s, b, pass == 1, Gp, ctx->stream),
+              "doubles (split fp32) kernel");
+      else
+        throw Fail{"unknown precision"};
+    }
+    if (ctx->timing) check(hipEventRecord(e1, ctx->stream), "event");
+    check(launch_energy(s, b, G, GJ, ijkabc ? -1.0 : 1.0, ctx->e_slot,
+                        ctx->stream),
+          "energy kernel");
+    if (ctx->timing) {
+      check(hipEventRecord(e2, ctx->stream), "event");
+      ctx->pending_events.push_back({e0, e1});
+      ctx->pending_names.push_back(dname);
+      ctx->pending_flop.push_back(flop_tuple * cnt * (use_j ? 2 : 1));
+      ctx->pending_events.push_back({e1, e2});
+      ctx->pending_names.push_back("energy");
+      ctx->pending_flop.push_back(0.0);
+    }
+    check(launch_batch_reduce(s, b, ctx->e_slot, ctx->partials, bi,
+                              ctx->stream),
+          "reduce kernel");
+    if (sink.active()) {
+      host_slots.resize(2 * b.slots);
+      check(hipMemcpyAsync(host_slots.data(), ctx->e_slot,
+                           host_slots.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream),
+            "download slots");
+      check(hipStreamSynchronize(ctx->stream), "slots");
+      sink(sel.data() + 3 * w0, b.nw, nb, host_slots);
+    }
+  }
+  std::vector<double> part(4 * nbatch);
+  check(hipMemcpyAsync(part.data(), ctx->partials, part.size() * sizeof(double),
+                       hipMemcpyDeviceToHost, ctx->stream),
+        "download partials");
+  check(hipStreamSynchronize(ctx->stream), "run");
+  // Neumaier over the batch partials in batch order
+  double s0 = 0, c0 = 0, s1 = 0, c1 = 0;
+  auto add = [](double &sum, double &comp, double x) {
+    const double t = sum + x;
+    comp += (std::fabs(sum) >= std::fabs(x)) ? (sum - t) + x : (x - t) + sum;
+    sum = t;
+  };
+  for (int bi = 0; bi < nbatch; bi++) {
+    add(s0, c0, part[4 * bi]);
+    add(s0, c0, part[4 * bi + 1]);
+    add(s1, c1, part[4 * bi + 2]);
+    add(s1, c1, part[4 * bi + 3]);
+  }
+  res.e = s0 + c0;
+  res.ect = s1 + c1;
+  // collect event timings
+  for (size_t q = 0; q < ctx->pending_events.size(); q++) {
+    float ms = 0.f;
+    check(hipEventElapsedTime(&ms, ctx->pending_events[q].first,
+                              ctx->pending_events[q].second),
+          "elapsed");
+    KernelStat &k = ctx->stats[ctx->pending_names[q]];
+    k.launches++;
+    k.ms += ms;
+    k.flop += ctx->pending_flop[q];
+  }
+  // each event appears as the end of one pair and the start of another
+  std::vector<hipEvent_t> uniq;
+  for (auto &pr : ctx->pending_events) {
+    uniq.push_back(pr.first);
+    uniq.push_back(pr.second);
+  }
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  for (auto ev : uniq) (void)hipEventDestroy(ev);
+  ctx->pending_events.clear();
+  ctx->pending_names.clear();
+  ctx->pending_flop.clear();
+  return res;
+}
+
+struct NoSink {
+  bool active() const { return false; }
+  void operator()(const int32_t *, int, int, const std::vector<double> &) {}
+};
+
+}  // namespace
+
+extern "C" {
+
+int atrip_hip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int atrip_hip_create(int device, atrip_hip_ctx **out) {
+  if (!out) return 1;
+  *out = nullptr;
+  auto *ctx = new atrip_hip_ctx();
+  ctx->device = device;
+  int rc = guarded(ctx, [&] {
+    int n = 0;
+    check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n)
+      throw Fail{"device index " + std::to_string(device) + " out of range (" +
+                 std::to_string(n) + " devices)"};
+    check(hipSetDevice(device), "hipSetDevice");
+    check(hipStreamCreate(&ctx->stream), "hipStreamCreate");
+  });
+  if (rc) {
+    static thread_local std::string last;
+    last = ctx->err;
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return 0;
+}
+
+int atrip_hip_destroy(atrip_hip_ctx *ctx) {
+  if (!ctx) return 0;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->systems) free_system(kv.second);
+  for (auto &kv : ctx->plans) (void)hipFree(kv.second.d_windows);
+  if (ctx->G) (void)hipFree(ctx->G);
+  if (ctx->e_slot) (void)hipFree(ctx->e_slot);
+  if (ctx->partials) (void)hipFree(ctx->partials);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+const char *atrip_hip_last_error(atrip_hip_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+int atrip_hip_upload(atrip_hip_ctx *ctx, const atrip_sys_desc *desc,
+                     int *sys) {
+  return guarded(ctx, [&] {
+    if (!desc || !sys) throw Fail{"null argument"};
+    if (!desc->ei || !desc->ea || !desc->Tai || !desc->Tabij ||
+        !desc->Vabij || !desc->Vijka || !desc->Vabci)
+      throw Fail{"missing input tensor"};
+    if ((desc->Jijka == nullptr) != (desc->Jabci == nullptr))
+      throw Fail{"Jijka and Jabci must be given together"};
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    DevSystem s;
+    try {
+      alloc_system(ctx, s, desc->No, desc->Nv, desc->Jijka != nullptr);
+      upload_host(ctx, s, desc);
+    } catch (...) {
+      free_system(s);
+      throw;
+    }
+    const int id = ctx->next_sys++;
+    ctx->systems.emplace(id, std::move(s));
+    *sys = id;
+  });
+}
+
+int atrip_hip_upload_synthetic(atrip_hip_ctx *ctx, int64_t No, int64_t Nv,
+                               uint64_t seed, int with_j, int *sys) {
+  return guarded(ctx, [&] {
+    if (!sys) throw Fail{"null argument"};
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    DevSystem s;
+    try {
+      alloc_system(ctx, s, No, Nv, with_j != 0);
+      check(launch_synth(s, seed, with_j != 0, ctx->stream), "synthetic fill");
+      check(hipStreamSynchronize(ctx->stream), "synthetic fill");
+    } catch (...) {
+      free_system(s);
+      throw;
+    }
+    const int id = ctx->next_sys++;
+    ctx->systems.emplace(id, std::move(s));
+    *sys = id;
+  });
+}
+
+int atrip_hip_release(atrip_hip_ctx *ctx, int sys) {
+  return guarded(ctx, [&] {
+    DevSystem &s = get_sys(ctx, sys);
+    check(hipStreamSynchronize(ctx->stream), "sync");
+    free_system(s);
+    ctx->systems.erase(sys);
+    ctx->plans.erase(sys);
+  });
+}
+
+int atrip_hip_run(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
+                  uint64_t first, uint64_t count, double *energy,
+                  double *ct_energy) {
+  return guarded(ctx, [&] {
+    if (!energy) throw Fail{"null energy pointer"};
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    const uint64_t last =
+        count == ~0ULL || first + count < first ? ~0ULL : first + count;
+    RunResult r = run_range(ctx, sys, precision, ijkabc, first, last, NoSink{});
+    // src/atrip/Atrip.cxx:1108-1111: E = -sum unless ijkabc
+    *energy = ijkabc ? r.e : -r.e;
+    if (ct_energy) *ct_energy = ijkabc ? r.ect : -r.ect;
+  });
+}
+
+int atrip_hip_tuple_energies(atrip_hip_ctx *ctx, int sys, int precision,
+                             int ijkabc, uint64_t first, uint64_t count,
+                             double *e, double *e_ct) {
+  return guarded(ctx, [&] {
+    if (!e) throw Fail{"null output pointer"};
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    DevSystem &s = get_sys(ctx, sys);
+    const uint64_t n = n_tuples(s.Nv);
+    if (first > n) first = n;
+    if (count > n - first) count = n - first;
+    const uint64_t last = first + count;
+    HostIndex hidx((uint64_t)s.Nv);
+    struct Sink {
+      bool active() const { return true; }
+      const HostIndex *h;
+      uint64_t first, last, Nv;
+      double *e, *ect;
+      void operator()(const int32_t *win, int nw, int nb,
+                      const std::vector<double> &slots) {
+        const int64_t nb3 = (int64_t)nb * nb * nb;
+        for (int w = 0; w < nw; w++)
+          for (int64_t q = 0; q < nb3; q++) {
+            const uint64_t a = (uint64_t)win[3 * w] * nb + q / (nb * nb);
+            const uint64_t b = (uint64_t)win[3 * w + 1] * nb + (q / nb) % nb;
+            const uint64_t c = (uint64_t)win[3 * w + 2] * nb + q % nb;
+            if (!tuple_ok(Nv, a, b, c)) continue;
+            const uint64_t idx = (*h)(a, b, c);
+            if (idx < first || idx >= last) continue;
+            e[idx - first] = slots[2 * (w * nb3 + q)];
+            if (ect) ect[idx - first] = slots[2 * (w * nb3 + q) + 1];
+          }
+      }
+    } sink{&hidx, first, last, (uint64_t)s.Nv, e, e_ct};
+    run_range(ctx, sys, precision, ijkabc, first, last, sink);
+  });
+}
+
+int atrip_hip_set_option(atrip_hip_ctx *ctx, const char *key, int64_t value) {
+  return guarded(ctx, [&] {
+    const std::string k = key ? key : "";
+    if (k == "window") {
+      if (value < 1 || value > 32) throw Fail{"window must be in 1..32"};
+      ctx->window = (int)value;
+    } else if (k == "batch") {
+      if (value < 1 || value > 4096) throw Fail{"batch must be in 1..4096"};
+      ctx->batch = (int)value;
+    } else if (k == "timing") {
+      ctx->timing = value != 0;
+    } else if (k == "tile") {
+      if (value < 0 || value > 2) throw Fail{"tile must be in 0..2"};
+      ctx->tile = (int)value;
+    } else {
+      throw Fail{"unknown option '" + k + "'"};
+    }
+  });
+}
+
+int atrip_hip_get_option(atrip_hip_ctx *ctx, const char *key, int64_t *value) {
+  return guarded(ctx, [&] {
+    const std::string k = key ? key : "";
+    if (!value) throw Fail{"null value pointer"};
+    if (k == "window") *value = ctx->window;
+    else if (k == "batch") *value = ctx->batch;
+    else if (k == "timing") *value = ctx->timing;
+    else if (k == "tile") *value = ctx->tile;
+    else throw Fail{"unknown option '" + k + "'"};
+  });
+}
+
+int atrip_hip_kernel_stats(atrip_hip_ctx *ctx, const char *name,
+                           uint64_t *launches, double *total_ms,
+                           double *flop) {
+  return guarded(ctx, [&] {
+    const KernelStat k = ctx->stats[name ? name : ""];
+    if (launches) *launches = k.launches;
+    if (total_ms) *total_ms = k.ms;
+    if (flop) *flop = k.flop;
+  });
+}
+
+int atrip_hip_reset_stats(atrip_hip_ctx *ctx) {
+  return guarded(ctx, [&] { ctx->stats.clear(); });
+}
+
+int atrip_hip_sys_info(atrip_hip_ctx *ctx, int sys, int64_t *No, int64_t *Nv,
+                       uint64_t *n_tup, uint64_t *device_bytes) {
+  return guarded(ctx, [&] {
+    DevSystem &s = get_sys(ctx, sys);
+    if (No) *No = s.No;
+    if (Nv) *Nv = s.Nv;
+    if (n_tup) *n_tup = n_tuples(s.Nv);
+    if (device_bytes) *device_bytes = s.bytes;
+  });
+}
+
+}  // extern "C"

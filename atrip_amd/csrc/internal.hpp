@@ -1,0 +1,143 @@
+// Internal declarations shared by the HIP translation units of libatrip_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/atrip_hip.h"
+
+namespace atrip_amd {
+
+// Contraction-index padding of the device layouts: every K run of the
+// doubles GEMMs is a multiple of the 16-element K chunk of the kernels.
+constexpr int KCHUNK = 16;
+inline int64_t pad16(int64_t n) { return (n + 15) / 16 * 16; }
+
+// Device-resident system.  Pair-indexed arrays use the CTF pair order
+// p(x,y) = x + Nv*y.  Layouts (all row-major with the last index fastest):
+//   T3  [x][p + No q][E < Kv]    = Tpphh[x,E,p,q]          (T(x) slices)
+//   V3  [p(x,y)][r][E < Kv]      = Vppph[x,y,E,r]          (V(xy) slices)
+//   H3  [x][p + No q][L < Ko]    = Vhhhp[p,q,L,x]          (Vhhh(x) slices)
+//   T2N [p(x,y)][n][L < Ko]      = -Tpphh[x,y,L,n]         (-T(xy)[L,n])
+//   T2TN[p(x,y)][n][L < Ko]      = -Tpphh[x,y,n,L]         (-T(xy)[n,L])
+//   VHH [p(x,y)][i + No j]       = Vpphh[x,y,i,j]          (V(xy)ij)
+//   Tai [a + Nv i], ei[i], ea[a] as in CTF.
+// Padding entries (E >= Nv, L >= No) are zero.
+struct DevSystem {
+  int64_t No = 0, Nv = 0, Kv = 0, Ko = 0;
+  double *T3 = nullptr, *V3 = nullptr, *H3 = nullptr, *T2N = nullptr,
+         *T2TN = nullptr, *VHH = nullptr, *Tai = nullptr, *ei = nullptr,
+         *ea = nullptr;
+  double *V3J = nullptr, *H3J = nullptr;  // (cT) integrals, may be null
+  uint32_t *points = nullptr;             // packed (i>=j>=k) triples
+  int64_t npoints = 0;
+  uint64_t *aprefix = nullptr;            // tuples with first index < a
+  uint64_t bytes = 0;
+  std::vector<void *> allocations;
+};
+
+// Tuple windows: block triples (alpha <= beta <= gamma) of edge nb.
+struct WindowPlan {
+  int nb = 0;
+  int nblk = 0;
+  std::vector<int32_t> windows;  // 3 per window, lexicographic
+  int32_t *d_windows = nullptr;
+};
+
+struct KernelStat {
+  uint64_t launches = 0;
+  double ms = 0.0;
+  double flop = 0.0;
+};
+
+struct Batch;
+
+}  // namespace atrip_amd
+
+struct atrip_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::map<int, atrip_amd::DevSystem> systems;
+  std::map<int, atrip_amd::WindowPlan> plans;  // keyed by system
+  int next_sys = 1;
+  // options
+  int window = 8;
+  int batch = 4;
+  int timing = 0;
+  int tile = 0;
+  // scratch, grown on demand
+  double *G = nullptr;       // class buffers [ncls][slots][No^3]
+  size_t G_bytes = 0;
+  double *e_slot = nullptr;  // per-slot energies (2 per slot: e, e_ct)
+  size_t e_slot_bytes = 0;
+  double *partials = nullptr;  // per-batch (sum, comp) x 2
+  size_t partials_bytes = 0;
+  std::map<std::string, atrip_amd::KernelStat> stats;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_events;
+  std::vector<std::string> pending_names;
+  std::vector<double> pending_flop;
+};
+
+namespace atrip_amd {
+
+// ---- launch wrappers (one translation unit each) -----------------------
+
+// Batch of consecutive windows handed to the hot kernels.
+struct BatchArgs {
+  const int32_t *windows;  // device, 3 per window of this batch
+  int nw;                  // windows in the batch
+  int nb;                  // window edge
+  uint64_t first, last;    // canonical tuple range [first, last)
+  int64_t slots;           // nw * nb^3
+};
+
+// Doubles contractions in FP64 MFMA: class buffers G[3][slots][No^3].
+hipError_t launch_doubles_f64(const DevSystem &s, const BatchArgs &b,
+                              bool use_j, double *G, int tile,
+                              hipStream_t st);
+
+// Doubles contractions in split FP32.
+hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
+                                bool use_j, double *G, hipStream_t st);
+
+// Singles + energy epilogue; writes e_slot[2*slot + {0,1}].
+hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
+                         const double *G, const double *GJ, double tai_sign,
+                         double *e_slot, hipStream_t st);
+
+// Deterministic compensated sum of the valid slot energies of a batch into
+// partials[4*index + {sum, comp, sum_ct, comp_ct}].
+hipError_t launch_batch_reduce(const DevSystem &s, const BatchArgs &b,
+                               const double *e_slot, double *partials,
+                               int index, hipStream_t st);
+
+// Upload helpers.
+hipError_t launch_transpose(const double *src, double *dst, int64_t n_outer,
+                            int64_t n_a, int64_t n_b, int64_t src_o,
+                            int64_t src_a, int64_t src_b, int64_t dst_o,
+                            int64_t dst_a, int64_t dst_b, double scale,
+                            hipStream_t st);
+hipError_t launch_synth(DevSystem &s, uint64_t seed, bool with_j,
+                        hipStream_t st);
+
+// Canonical (lexicographic) index of tuple (a,b,c), a<=b<=c, not all equal.
+__host__ __device__ inline uint64_t tuple_index(const uint64_t *aprefix,
+                                                uint64_t Nv, uint64_t a,
+                                                uint64_t b, uint64_t c) {
+  // tuples (a, b', c') with a <= b' < b: sum_{b'=a}^{b-1} (Nv - b')
+  const uint64_t nb = b - a;
+  const uint64_t before_b = nb * Nv - (nb * (a + b - 1)) / 2;
+  return aprefix[a] + before_b - (b > a ? 1 : 0) + (c - b) - (a == b ? 1 : 0);
+}
+
+__host__ __device__ inline bool tuple_ok(uint64_t Nv, uint64_t a, uint64_t b,
+                                         uint64_t c) {
+  return a <= b && b <= c && c < Nv && !(a == b && b == c);
+}
+
+}  // namespace atrip_amd

@@ -1,0 +1,211 @@
+"""ctypes binding of libatrip_hip.so (include/atrip_hip.h).
+
+The product path: every call goes to the HIP library; there is no CPU
+fallback.  A missing library raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libatrip_hip.so")
+
+FP64 = 0
+SPLIT_FP32 = 1
+PRECISIONS = {"fp64": FP64, "split_fp32": SPLIT_FP32, "mixed": SPLIT_FP32}
+
+# Exported C-ABI symbols (kept in sync with include/atrip_hip.h).
+SYMBOLS = (
+    "atrip_hip_create", "atrip_hip_destroy", "atrip_hip_last_error",
+    "atrip_hip_upload", "atrip_hip_upload_synthetic", "atrip_hip_release",
+    "atrip_hip_run", "atrip_hip_tuple_energies", "atrip_hip_set_option",
+    "atrip_hip_get_option", "atrip_hip_kernel_stats", "atrip_hip_reset_stats",
+    "atrip_hip_sys_info", "atrip_hip_device_count",
+)
+
+TENSORS = ("ei", "ea", "Tai", "Tabij", "Vabij", "Vijka", "Vabci", "Jijka", "Jabci")
+
+
+class HipError(RuntimeError):
+    pass
+
+
+class SysDesc(C.Structure):
+    _fields_ = [("No", C.c_int64), ("Nv", C.c_int64)] + [
+        (name, C.POINTER(C.c_double)) for name in TENSORS
+    ]
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HipError(
+            f"{LIB_PATH} is missing: build it with `make -C atrip_amd` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.atrip_hip_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.atrip_hip_destroy.argtypes = [vp]
+    L.atrip_hip_last_error.argtypes = [vp]
+    L.atrip_hip_last_error.restype = C.c_char_p
+    L.atrip_hip_upload.argtypes = [vp, C.POINTER(SysDesc), C.POINTER(C.c_int)]
+    L.atrip_hip_upload_synthetic.argtypes = [
+        vp, C.c_int64, C.c_int64, C.c_uint64, C.c_int, C.POINTER(C.c_int)]
+    L.atrip_hip_release.argtypes = [vp, C.c_int]
+    L.atrip_hip_run.argtypes = [
+        vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
+        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.atrip_hip_tuple_energies.argtypes = [
+        vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
+        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.atrip_hip_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
+    L.atrip_hip_get_option.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
+    L.atrip_hip_kernel_stats.argtypes = [
+        vp, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+        C.POINTER(C.c_double)]
+    L.atrip_hip_reset_stats.argtypes = [vp]
+    L.atrip_hip_sys_info.argtypes = [
+        vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.atrip_hip_device_count.argtypes = []
+    L.atrip_hip_device_count.restype = C.c_int
+    _lib = L
+    return L
+
+
+def device_count() -> int:
+    return int(load().atrip_hip_device_count())
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@dataclass
+class KernelStats:
+    launches: int
+    total_ms: float
+    flop: float
+
+    @property
+    def avg_ms(self) -> float:
+        return self.total_ms / self.launches if self.launches else 0.0
+
+
+class Context:
+    """One HIP device context (not thread-safe; one host thread each)."""
+
+    def __init__(self, device: int = 0):
+        self._L = load()
+        self._h = C.c_void_p()
+        rc = self._L.atrip_hip_create(device, C.byref(self._h))
+        if rc != 0:
+            raise HipError(f"atrip_hip_create(device={device}) failed ({rc})")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._L.atrip_hip_last_error(self._h).decode()
+            raise HipError(f"{what}: {msg}")
+
+    def close(self):
+        if self._h:
+            self._L.atrip_hip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- systems ---------------------------------------------------------
+    def upload(self, no: int, nv: int, t: dict) -> int:
+        """Upload CTF-layout float64 arrays (flat, first index fastest)."""
+        d = SysDesc()
+        d.No, d.Nv = no, nv
+        keep = []
+        for name in TENSORS:
+            arr = t.get(name)
+            if arr is None:
+                setattr(d, name, C.POINTER(C.c_double)())
+                continue
+            arr = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+            keep.append(arr)
+            setattr(d, name, _p(arr))
+        sys_ = C.c_int()
+        self._check(self._L.atrip_hip_upload(self._h, C.byref(d), C.byref(sys_)),
+                    "upload")
+        return sys_.value
+
+    def upload_synthetic(self, no: int, nv: int, seed: int, with_j=False) -> int:
+        sys_ = C.c_int()
+        self._check(self._L.atrip_hip_upload_synthetic(
+            self._h, no, nv, seed, int(with_j), C.byref(sys_)), "upload_synthetic")
+        return sys_.value
+
+    def release(self, sys_: int):
+        self._check(self._L.atrip_hip_release(self._h, sys_), "release")
+
+    def sys_info(self, sys_: int):
+        no, nv = C.c_int64(), C.c_int64()
+        nt, nbytes = C.c_uint64(), C.c_uint64()
+        self._check(self._L.atrip_hip_sys_info(
+            self._h, sys_, C.byref(no), C.byref(nv), C.byref(nt), C.byref(nbytes)),
+            "sys_info")
+        return dict(No=no.value, Nv=nv.value, n_tuples=nt.value, bytes=nbytes.value)
+
+    # -- runs ------------------------------------------------------------
+    def run(self, sys_: int, precision="fp64", ijkabc=False, first=0, count=None):
+        e, ect = C.c_double(), C.c_double()
+        cnt = (1 << 64) - 1 if count is None else count
+        self._check(self._L.atrip_hip_run(
+            self._h, sys_, PRECISIONS[precision], int(ijkabc), first, cnt,
+            C.byref(e), C.byref(ect)), "run")
+        return e.value, ect.value
+
+    def tuple_energies(self, sys_: int, first: int, count: int, precision="fp64",
+                       ijkabc=False, with_ct=False):
+        e = np.zeros(max(count, 1))
+        ect = np.zeros(max(count, 1)) if with_ct else None
+        self._check(self._L.atrip_hip_tuple_energies(
+            self._h, sys_, PRECISIONS[precision], int(ijkabc), first, count,
+            _p(e), _p(ect) if with_ct else C.POINTER(C.c_double)()),
+            "tuple_energies")
+        return (e[:count], ect[:count]) if with_ct else e[:count]
+
+    # -- knobs -----------------------------------------------------------
+    def set_option(self, key: str, value: int):
+        self._check(self._L.atrip_hip_set_option(self._h, key.encode(), int(value)),
+                    f"set_option({key})")
+
+    def get_option(self, key: str) -> int:
+        v = C.c_int64()
+        self._check(self._L.atrip_hip_get_option(self._h, key.encode(), C.byref(v)),
+                    f"get_option({key})")
+        return v.value
+
+    def kernel_stats(self, name: str) -> KernelStats:
+        n, ms, fl = C.c_uint64(), C.c_double(), C.c_double()
+        self._check(self._L.atrip_hip_kernel_stats(
+            self._h, name.encode(), C.byref(n), C.byref(ms), C.byref(fl)),
+            "kernel_stats")
+        return KernelStats(n.value, ms.value, fl.value)
+
+    def reset_stats(self):
+        self._check(self._L.atrip_hip_reset_stats(self._h), "reset_stats")
