@@ -208,36 +208,6 @@ DevSystem &get_sys(atrip_hip_ctx *ctx, int sys) {
   return it->second;
 }
 
-// Window plan for a system at the context's window edge.
-WindowPlan &get_plan(atrip_hip_ctx *ctx, int sys, const DevSystem &s) {
-  auto it = ctx->plans.find(sys);
-  if (it != ctx->plans.end() && it->second.nb == ctx->window) return it->second;
-  if (it != ctx->plans.end()) {
-    (void)hipFree(it->second.d_windows);
-    ctx->plans.erase(it);
-  }
-  WindowPlan p;
-  p.nb = ctx->window;
-  p.nblk = (int)((s.Nv + p.nb - 1) / p.nb);
-  for (int al = 0; al < p.nblk; al++)
-    for (int be = al; be < p.nblk; be++)
-      for (int ga = be; ga < p.nblk; ga++) {
-        p.windows.push_back(al);
-        p.windows.push_back(be);
-        p.windows.push_back(ga);
-      }
-  return ctx->plans.emplace(sys, std::move(p)).first->second;
-}
-
-void ensure(void *&ptr, size_t &have, size_t need, const char *what) {
-  if (have >= need && ptr) return;
-  if (ptr) check(hipFree(ptr), what);
-  ptr = nullptr;
-  have = 0;
-  check(hipMalloc(&ptr, need), what);
-  have = need;
-}
-
 // Host-side canonical index, identical to the device one.
 struct HostIndex {
   std::vector<uint64_t> pre;
@@ -252,6 +222,51 @@ struct HostIndex {
     return tuple_index(pre.data(), Nv, a, b, c);
   }
 };
+
+// Window plan for a system at the context's window edge: the block triples
+// in lexicographic order with their canonical index span and tuple count.
+WindowPlan &get_plan(atrip_hip_ctx *ctx, int sys, const DevSystem &s) {
+  auto it = ctx->plans.find(sys);
+  if (it != ctx->plans.end() && it->second.nb == ctx->window) return it->second;
+  if (it != ctx->plans.end()) ctx->plans.erase(it);
+  WindowPlan p;
+  p.nb = ctx->window;
+  p.nblk = (int)((s.Nv + p.nb - 1) / p.nb);
+  const HostIndex hidx((uint64_t)s.Nv);
+  const int nb = p.nb;
+  for (int al = 0; al < p.nblk; al++)
+    for (int be = al; be < p.nblk; be++)
+      for (int ga = be; ga < p.nblk; ga++) {
+        uint64_t lo = ~0ULL, hi = 0, cnt = 0;
+        for (int ta = 0; ta < nb; ta++)
+          for (int tb = 0; tb < nb; tb++)
+            for (int tc = 0; tc < nb; tc++) {
+              const uint64_t a = (uint64_t)al * nb + ta,
+                             b = (uint64_t)be * nb + tb,
+                             c = (uint64_t)ga * nb + tc;
+              if (!tuple_ok(s.Nv, a, b, c)) continue;
+              const uint64_t q = hidx(a, b, c);
+              lo = std::min(lo, q);
+              hi = std::max(hi, q);
+              cnt++;
+            }
+        if (!cnt) continue;
+        p.windows.insert(p.windows.end(), {al, be, ga});
+        p.wmin.push_back(lo);
+        p.wmax.push_back(hi);
+        p.wcount.push_back(cnt);
+      }
+  return ctx->plans.emplace(sys, std::move(p)).first->second;
+}
+
+void ensure(void *&ptr, size_t &have, size_t need, const char *what) {
+  if (have >= need && ptr) return;
+  if (ptr) check(hipFree(ptr), what);
+  ptr = nullptr;
+  have = 0;
+  check(hipMalloc(&ptr, need), what);
+  have = need;
+}
 
 struct RunResult {
   double e = 0, ect = 0;
@@ -279,17 +294,21 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
     const int64_t al = plan.windows[3 * w], be = plan.windows[3 * w + 1],
                   ga = plan.windows[3 * w + 2];
     uint64_t cnt = 0;
-    for (int ta = 0; ta < nb; ta++)
-      for (int tb = 0; tb < nb; tb++)
-        for (int tc = 0; tc < nb; tc++) {
-          const uint64_t a = al * nb + ta, b = be * nb + tb, c = ga * nb + tc;
-          if (!tuple_ok(s.Nv, a, b, c)) continue;
-          if (!full) {
+    if (full || (plan.wmin[w] >= first && plan.wmax[w] < last)) {
+      cnt = plan.wcount[w];
+    } else if (plan.wmax[w] < first || plan.wmin[w] >= last) {
+      continue;
+    } else {
+      for (int ta = 0; ta < nb; ta++)
+        for (int tb = 0; tb < nb; tb++)
+          for (int tc = 0; tc < nb; tc++) {
+            const uint64_t a = al * nb + ta, b = be * nb + tb,
+                           c = ga * nb + tc;
+            if (!tuple_ok(s.Nv, a, b, c)) continue;
             const uint64_t q = hidx(a, b, c);
-            if (q < first || q >= last) continue;
+            if (q >= first && q < last) cnt++;
           }
-          cnt++;
-        }
+    }
     if (!cnt) continue;
     sel.insert(sel.end(), {(int32_t)al, (int32_t)be, (int32_t)ga});
     counts.push_back(cnt);
@@ -469,7 +488,6 @@ int atrip_hip_destroy(atrip_hip_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto &kv : ctx->systems) free_system(kv.second);
-  for (auto &kv : ctx->plans) (void)hipFree(kv.second.d_windows);
   if (ctx->G) (void)hipFree(ctx->G);
   if (ctx->e_slot) (void)hipFree(ctx->e_slot);
   if (ctx->partials) (void)hipFree(ctx->partials);

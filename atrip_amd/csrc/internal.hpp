@@ -45,7 +45,8 @@ struct WindowPlan {
   int nb = 0;
   int nblk = 0;
   std::vector<int32_t> windows;  // 3 per window, lexicographic
-  int32_t *d_windows = nullptr;
+  // per window: smallest and largest canonical index and number of tuples
+  std::vector<uint64_t> wmin, wmax, wcount;
 };
 
 struct KernelStat {

@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark of the (T) perturbative-triples hot path on one MI355X.
+
+Workload: the plain FP64 (T) energy of a synthetic closed-shell system with
+No=40 occupied and Nv=400 virtual orbitals (10,746,400 tuples a<=b<=c), all
+tensors generated on the device from a seed and resident in HBM before the
+timed region starts.  A step is one slice of the tuple space: the tuples whose
+first index falls in a contiguous range of index blocks, with the slices
+balanced so that 16 steps cover the whole space once.  The default run
+(16 timed steps) therefore computes the full E(T) of the system.
+
+Prints one JSON line with tuples/s, TFLOP/s (doubles FLOP definition of the
+reference, 12 No^3 (No+Nv) per tuple), the FP64-MFMA roofline of the doubles
+kernel measured with HIP events on the stream it runs on, and a CPU baseline
+(the oracle's port of the reference CPU path on the host cores, a bounded
+sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = ("(T) tuples/s & TFLOP/s, FP64 vs mixed @|ΔE|≤1e-7 Eh; "
+          "serve p50/p99")
+PEAK_TFLOPS = {"fp64": 78.6, "split_fp32": 157.3}
+
+
+def n_tuples(nv: int) -> int:
+    return nv * (nv + 1) * (nv + 2) // 6 - nv
+
+
+def tuples_before(a: int, nv: int) -> int:
+    """Tuples whose first index is < a (lexicographic order)."""
+    tot = 0
+    for x in range(a):
+        m = nv - x
+        tot += m * (m + 1) // 2 - 1
+    return tot
+
+
+def step_slices(nv: int, nb: int, nsteps: int):
+    """Canonical [first, count) ranges aligned to index blocks of edge nb,
+    balanced by tuple count, covering the whole tuple space."""
+    nblk = (nv + nb - 1) // nb
+    bounds = [tuples_before(min(b * nb, nv), nv) for b in range(nblk + 1)]
+    total = bounds[-1]
+    cuts = [0]
+    for s in range(1, nsteps):
+        target = total * s / nsteps
+        b = min(range(len(bounds)), key=lambda q: abs(bounds[q] - target))
+        cuts.append(max(b, cuts[-1]))
+    cuts.append(nblk)
+    out = []
+    for s in range(nsteps):
+        lo, hi = bounds[cuts[s]], bounds[cuts[s + 1]]
+        if hi > lo:
+            out.append((lo, hi - lo))
+    return out
+
+
+def cpu_baseline(no: int, nv: int, seed: int, budget_s: float, cores: int):
+    """Oracle (C port of the reference np=1 CPU path, host BLAS dgemm) on the
+    first tuples of the same workload, all host cores as tuple shards."""
+    from oracle import oracle as O  # the checker / CPU baseline only
+
+    blas = O.blas_init()
+    t0 = time.time()
+    t = O.synthetic(no, nv, seed, nthreads=cores)
+    gen_s = time.time() - t0
+    flop_tuple = 12.0 * no ** 3 * (no + nv)
+    # calibrate on a small sample, then size the measured sample to the budget
+    probe = max(cores, 4)
+    t0 = time.time()
+    O.run(no, nv, t, first=0, count=probe, use_gemm=True, nthreads=cores)
+    dt = max(time.time() - t0, 1e-3)
+    count = int(max(probe, min(n_tuples(nv), probe * budget_s / dt)))
+    count = (count // cores) * cores or cores
+    t0 = time.time()
+    O.run(no, nv, t, first=0, count=count, use_gemm=True, nthreads=cores)
+    dt = time.time() - t0
+    rate = count / dt
+    return {
+        "value": rate,
+        "unit": "tuples/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"first {count} tuples (lexicographic order) of No={no} "
+                   f"Nv={nv} seed={seed}; oracle port with dgemm from "
+                   f"{blas or 'plain C'}; {cores} tuple-shard threads, "
+                   f"single-threaded BLAS each; {dt:.1f} s measured, "
+                   f"{gen_s:.1f} s host generation untimed"),
+        "tflops": rate * flop_tuple / 1e12,
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no", type=int, default=40)
+    ap.add_argument("--nv", type=int, default=400)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "split_fp32"])
+    ap.add_argument("--window", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--slices", type=int, default=16,
+                    help="steps that cover the tuple space once")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args(argv)
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(tuples would shard by index range with one scalar all-reduce; "
+              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    from atrip_amd import hip as H
+
+    ctx = H.Context(0)
+    ctx.set_option("window", args.window)
+    ctx.set_option("batch", args.batch)
+    ctx.set_option("tile", args.tile)
+    t0 = time.time()
+    sys_ = ctx.upload_synthetic(args.no, args.nv, args.seed)
+    gen_s = time.time() - t0
+    slices = step_slices(args.nv, args.window, args.slices)
+
+    for i in range(args.warmup):
+        first, count = slices[(len(slices) - 1 - i) % len(slices)]
+        ctx.run(sys_, args.precision, first=first, count=count)
+
+    ctx.reset_stats()
+    ctx.set_option("timing", 1)
+    done = 0
+    e_sum = 0.0
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        first, count = slices[i % len(slices)]
+        e, _ = ctx.run(sys_, args.precision, first=first, count=count)
+        e_sum += e
+        done += count
+    t_end = time.perf_counter()
+    elapsed = t_end - t_start
+    ctx.set_option("timing", 0)
+
+    flop_tuple = 12.0 * args.no ** 3 * (args.no + args.nv)
+    kname = "doubles" if args.precision == "fp64" else "doubles_mixed"
+    ks = ctx.kernel_stats(kname)
+    es = ctx.kernel_stats("energy")
+    achieved = ks.flop / (ks.total_ms * 1e-3) / 1e12 if ks.total_ms else 0.0
+    peak = PEAK_TFLOPS[args.precision]
+    value = done / elapsed
+
+    cpu = None
+    if not args.no_cpu:
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        cpu = cpu_baseline(args.no, args.nv, args.seed, args.cpu_seconds, cores)
+
+    full = args.steps % len(slices) == 0 and args.steps > 0
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "tuples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32x2(split)+f64",
+        "data": "synthetic (seeded counter-based generator, generated on device)",
+        "config": {
+            "workload": (f"plain (T) No={args.no} Nv={args.nv}, all tuples "
+                         f"a<=b<=c ({n_tuples(args.nv)}), {args.precision}"),
+            "No": args.no, "Nv": args.nv, "seed": args.seed,
+            "precision": args.precision, "window": args.window,
+            "batch": args.batch, "tile": args.tile,
+            "parallelism": "single GPU",
+        },
+        "tflops": value * flop_tuple / 1e12,
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": None,
+            "kernel": kname,
+            "launches": ks.launches,
+            "avg_launch_ms": ks.avg_ms,
+        },
+        "energy_kernel_ms": es.total_ms,
+        "tuples_timed": done,
+        "energy_Eh": e_sum if full else None,
+        "setup_s": gen_s,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -131,9 +131,10 @@ int oracle_run(const oracle_input *in, int ijkabc, uint64_t first,
                double *per_tuple_ct);
 
 /* Fill a whole tensor in CTF layout from the counter-based generator
- * (include/atrip_synth.h).  n = number of elements of that tensor. */
+ * (include/atrip_synth.h).  n = number of elements of that tensor; the
+ * elements are split over nthreads threads. */
 void oracle_fill_synthetic(uint64_t seed, int tensor, size_t No, size_t Nv,
-                           double *out, size_t n);
+                           double *out, size_t n, int nthreads);
 
 /* Host BLAS used by the GEMM form: loads dgemm_ from `path` (or searches
  * a few standard names when NULL).  Returns the library name, or NULL when

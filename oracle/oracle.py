@@ -56,6 +56,7 @@ def lib():
         ]
         L.oracle_fill_synthetic.argtypes = [
             C.c_uint64, C.c_int, C.c_size_t, C.c_size_t, C.POINTER(C.c_double), C.c_size_t,
+            C.c_int,
         ]
         L.oracle_blas_init.restype = C.c_char_p
         L.oracle_blas_init.argtypes = [C.c_char_p]
@@ -103,14 +104,14 @@ def tensor_size(name: str, no: int, nv: int) -> int:
     }[name]
 
 
-def synthetic(no: int, nv: int, seed: int, with_j: bool = False) -> dict:
+def synthetic(no: int, nv: int, seed: int, with_j: bool = False, nthreads: int = 1) -> dict:
     """Seeded synthetic inputs (include/atrip_synth.h), CTF layout, flat."""
     out = {}
     names = TENSORS if with_j else TENSORS[:7]
     for name in names:
         n = tensor_size(name, no, nv)
         arr = np.empty(n, dtype=np.float64)
-        lib().oracle_fill_synthetic(seed, TENSOR_ID[name], no, nv, _p(arr), n)
+        lib().oracle_fill_synthetic(seed, TENSOR_ID[name], no, nv, _p(arr), n, nthreads)
         out[name] = arr
     return out
 

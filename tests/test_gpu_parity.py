@@ -1,0 +1,162 @@
+"""Parity of the HIP path against the CPU oracle, through the C-ABI.
+
+Bar: per-tuple energies within 1e-12 of the largest per-tuple magnitude and
+total E(T) within 1e-10 relative (the plain FP64 tolerance), on the same
+seeded inputs; bitwise identity where the arithmetic is the same (device vs
+host generation, window/batch partitioning).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+REL_E = 1e-10       # total E(T), relative
+REL_TUPLE = 1e-12   # per tuple, relative to max |E_abc|
+FLOOR = 1e-18       # absolute, Eh
+
+
+def _check(ctx, oracle, no, nv, seed, window=8, batch=4, with_j=False):
+    t = oracle.synthetic(no, nv, seed, with_j=with_j)
+    e_ref, ect_ref, pt_ref, ptc_ref = oracle.run(no, nv, t, per_tuple=True)
+    ctx.set_option("window", window)
+    ctx.set_option("batch", batch)
+    s = ctx.upload(no, nv, t)
+    try:
+        e, ect = ctx.run(s)
+        n = oracle.n_tuples(nv)
+        pt, ptc = ctx.tuple_energies(s, 0, n, with_ct=True)
+    finally:
+        ctx.release(s)
+    # No = 1 has E = 0 in exact arithmetic: FLOOR absorbs pure roundoff
+    scale = np.max(np.abs(pt_ref))
+    assert np.max(np.abs(pt - pt_ref)) <= REL_TUPLE * scale + FLOOR
+    assert np.max(np.abs(ptc - ptc_ref)) <= REL_TUPLE * np.max(np.abs(ptc_ref)) + FLOOR
+    assert abs(e - e_ref) <= REL_E * abs(e_ref) + FLOOR
+    assert abs(ect - ect_ref) <= REL_E * abs(ect_ref) + FLOOR
+    return e, pt
+
+
+@pytest.mark.parametrize("no,nv,window,batch", [
+    (1, 3, 2, 1), (2, 2, 8, 4), (3, 4, 2, 3), (4, 7, 3, 2), (5, 9, 8, 1),
+    (7, 13, 4, 5), (16, 17, 8, 2), (17, 10, 3, 4), (9, 33, 16, 1), (20, 24, 8, 4),
+])
+def test_small_systems(hip_ctx, oracle, no, nv, window, batch):
+    _check(hip_ctx, oracle, no, nv, seed=100 + no * nv, window=window, batch=batch)
+
+
+def test_c1_full_golden(hip_ctx, oracle):
+    """No=10, Nv=50, all 22,050 tuples: the CPU-runnable configuration."""
+    gold = json.load(open(os.path.join(GOLDEN, "c1_seed1.json")))
+    e, pt = _check(hip_ctx, oracle, 10, 50, seed=1)
+    assert abs(e - gold["energy"]) <= REL_E * abs(gold["energy"])
+    for idx, val in gold["tuple_energies"].items():
+        assert abs(pt[int(idx)] - val) <= REL_TUPLE * gold["max_abs_tuple_energy"]
+
+
+def test_cT_pass(hip_ctx, oracle):
+    _check(hip_ctx, oracle, 5, 11, seed=21, window=4, with_j=True)
+
+
+def test_ijkabc(hip_ctx, oracle):
+    no, nv = 6, 12
+    t = oracle.synthetic(no, nv, 31)
+    e_ref, _ = oracle.run(no, nv, t, ijkabc=True)
+    s = hip_ctx.upload(no, nv, t)
+    try:
+        e, ect = hip_ctx.run(s, ijkabc=True)
+    finally:
+        hip_ctx.release(s)
+    assert abs(e - e_ref) <= REL_E * abs(e_ref) and ect == e
+
+
+@pytest.mark.parametrize("first,count", [(0, 1), (0, 100), (37, 411), (700, 10 ** 9),
+                                         (1000, 0)])
+def test_ranges(hip_ctx, oracle, first, count):
+    no, nv = 6, 20
+    t = oracle.synthetic(no, nv, 41)
+    n = oracle.n_tuples(nv)
+    cnt = max(0, min(count, n - first))
+    s = hip_ctx.upload(no, nv, t)
+    try:
+        hip_ctx.set_option("window", 4)
+        e, _ = hip_ctx.run(s, first=first, count=count)
+        pt = hip_ctx.tuple_energies(s, first, cnt) if cnt else np.zeros(0)
+    finally:
+        hip_ctx.release(s)
+    if cnt == 0:
+        assert e == 0.0
+        return
+    e_ref, _, pt_ref, _ = oracle.run(no, nv, t, first=first, count=cnt, per_tuple=True)
+    assert abs(e - e_ref) <= REL_E * abs(e_ref)
+    assert np.max(np.abs(pt - pt_ref)) <= REL_TUPLE * np.max(np.abs(pt_ref))
+
+
+def test_device_generation_is_bit_identical(hip_ctx, oracle):
+    no, nv, seed = 11, 23, 5
+    t = oracle.synthetic(no, nv, seed, with_j=True)
+    s1 = hip_ctx.upload(no, nv, t)
+    s2 = hip_ctx.upload_synthetic(no, nv, seed, with_j=True)
+    try:
+        n = oracle.n_tuples(nv)
+        a, ac = hip_ctx.tuple_energies(s1, 0, n, with_ct=True)
+        b, bc = hip_ctx.tuple_energies(s2, 0, n, with_ct=True)
+    finally:
+        hip_ctx.release(s1)
+        hip_ctx.release(s2)
+    assert np.array_equal(a, b) and np.array_equal(ac, bc)
+
+
+def test_partition_independence(hip_ctx, oracle):
+    """Per-tuple energies do not depend on which tuples share a launch."""
+    no, nv = 12, 40
+    s = hip_ctx.upload_synthetic(no, nv, 77)
+    try:
+        n = oracle.n_tuples(nv)
+        outs = []
+        for window, batch, tile in [(8, 4, 0), (4, 1, 0), (16, 2, 1), (5, 7, 2)]:
+            hip_ctx.set_option("window", window)
+            hip_ctx.set_option("batch", batch)
+            hip_ctx.set_option("tile", tile)
+            outs.append(hip_ctx.tuple_energies(s, 0, n))
+        hip_ctx.set_option("tile", 0)
+        hip_ctx.set_option("window", 8)
+        hip_ctx.set_option("batch", 4)
+        # a single tuple alone gives the same bits as in the full run
+        alone = hip_ctx.tuple_energies(s, 1234, 1)
+    finally:
+        hip_ctx.release(s)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    assert alone[0] == outs[0][1234]
+
+
+def test_mid_size_sample(hip_ctx, oracle):
+    """No=24, Nv=120 (Vabci 330 MB): sampled tuple ranges against the oracle."""
+    no, nv, seed = 24, 120, 3
+    t = oracle.synthetic(no, nv, seed, nthreads=8)
+    oracle.blas_init()
+    s = hip_ctx.upload_synthetic(no, nv, seed)
+    try:
+        for first, count in [(0, 300), (150000, 200), (oracle.n_tuples(nv) - 250, 250)]:
+            pt = hip_ctx.tuple_energies(s, first, count)
+            _, _, pt_ref, _ = oracle.run(no, nv, t, first=first, count=count,
+                                         use_gemm=True, nthreads=8, per_tuple=True)
+            assert np.max(np.abs(pt - pt_ref)) <= REL_TUPLE * np.max(np.abs(pt_ref))
+    finally:
+        hip_ctx.release(s)
+
+
+def test_errors_are_reported(hip_ctx):
+    from atrip_amd import hip
+
+    with pytest.raises(hip.HipError, match="unknown system"):
+        hip_ctx.run(987654)
+    with pytest.raises(hip.HipError, match="window"):
+        hip_ctx.set_option("window", 0)
+    with pytest.raises(hip.HipError, match="unknown option"):
+        hip_ctx.set_option("no-such-knob", 1)
