@@ -1,0 +1,43 @@
+"""Runs the C++ drop-in tests (tests/cxx/test_main.cxx) and the `atrip` CLI."""
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cxx", "_build", "test_main")
+CLI = os.path.join(ROOT, "atrip_amd", "atrip")
+
+
+def _need(path):
+    if not os.path.exists(path):
+        pytest.fail(f"{path} missing: run __graft_entry__.build()")
+
+
+def test_cxx_host_cases():
+    _need(BIN)
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "host cases passed" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cxx_run_cases():
+    _need(BIN)
+    r = subprocess.run([BIN, "--gpu"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "all cases passed" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cli_energy_matches_oracle(oracle, tmp_path):
+    _need(CLI)
+    no, nv, seed = 7, 15, 3
+    r = subprocess.run([CLI, "--no", str(no), "--nv", str(nv), "--seed", str(seed),
+                        "--nocheckpoint"], capture_output=True, text=True, timeout=300,
+                       cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    e = float(re.search(r"^Energy: (\S+)", r.stdout, re.M).group(1))
+    e_ref, _ = oracle.run(no, nv, oracle.synthetic(no, nv, seed))
+    assert abs(e - e_ref) <= 1e-10 * abs(e_ref)
