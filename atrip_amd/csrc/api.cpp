@@ -328,9 +328,9 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   const int bw = std::max(1, ctx->batch);
   const int64_t nb3 = (int64_t)nb * nb * nb;
   const int64_t slots_max = (int64_t)std::min(bw, nsel) * nb3;
-  const int64_t N3 = s.No * s.No * s.No;
+  const int64_t W3 = w_pad(s.No) * w_pad(s.No) * w_pad(s.No);
   const bool use_j = s.V3J != nullptr;
-  const size_t gsz = (size_t)(3 * slots_max * N3) * sizeof(double);
+  const size_t gsz = (size_t)(3 * slots_max * W3) * sizeof(double);
   ensure((void *&)ctx->G, ctx->G_bytes, use_j ? 2 * gsz : gsz, "alloc G");
   ensure((void *&)ctx->e_slot, ctx->e_slot_bytes,
          (size_t)(2 * slots_max) * sizeof(double), "alloc e_slot");
@@ -338,7 +338,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   ensure((void *&)ctx->partials, ctx->partials_bytes,
          (size_t)(4 * nbatch) * sizeof(double), "alloc partials");
   double *G = ctx->G;
-  double *GJ = use_j ? ctx->G + 3 * slots_max * N3 : nullptr;
+  double *GJ = use_j ? ctx->G + 3 * slots_max * W3 : nullptr;
   const double flop_tuple = 12.0 * (double)s.No * s.No * s.No * (s.No + s.Nv);
   const char *dname = precision == ATRIP_FP64 ? "doubles" : "doubles_mixed";
   std::vector<double> host_slots;
@@ -618,7 +618,7 @@ int atrip_hip_set_option(atrip_hip_ctx *ctx, const char *key, int64_t value) {
     } else if (k == "timing") {
       ctx->timing = value != 0;
     } else if (k == "tile") {
-      if (value < 0 || value > 2) throw Fail{"tile must be in 0..2"};
+      if (value < -1 || value > 7) throw Fail{"tile must be in -1..7"};
       ctx->tile = (int)value;
     } else {
       throw Fail{"unknown option '" + k + "'"};

@@ -19,6 +19,8 @@
 // resident) into registers.  A K chunk of 16 is split as
 // kappa = k0 + 4*(lane/16) + s, s = 0..3, so every lane reads 4 consecutive
 // doubles (32 B) of one row/column per chunk and feeds 4 MFMA k-steps.
+#include <cmath>
+
 #include "internal.hpp"
 
 namespace atrip_amd {
@@ -31,7 +33,7 @@ struct DoublesParams {
   const int32_t *windows;
   const uint64_t *aprefix;
   double *G;
-  int64_t No, Nv, Kv, Ko, N2, N3, slots;
+  int64_t No, Nv, Kv, Ko, N2, W3, nbk, slots;
   uint64_t first, last;
   int nb, mtiles, ntiles;
   int64_t total, per_xcd;
@@ -51,6 +53,30 @@ __device__ inline void class_tuple(int cl, int64_t al, int64_t be, int64_t ga,
     a = al * nb + t; b = be * nb + o1; c = ga * nb + o2;
     slot = ((int64_t)t * nb + o1) * nb + o2;
   }
+}
+
+// B operand of segment sg for a column tuple: table of the file comment.
+__device__ inline const double *b_base(const DoublesParams &P, int cl, int sg,
+                                       int64_t a, int64_t b, int64_t c) {
+  int64_t u, v;
+  const double *Bt;
+  if (cl == 0) {
+    if (sg == 0) { u = b; v = c; Bt = P.V3; }
+    else if (sg == 1) { u = a; v = c; Bt = P.V3; }
+    else if (sg == 2) { u = a; v = c; Bt = P.T2N; }
+    else { u = b; v = c; Bt = P.T2N; }
+  } else if (cl == 1) {
+    if (sg == 0) { u = c; v = b; Bt = P.V3; }
+    else if (sg == 1) { u = a; v = b; Bt = P.V3; }
+    else if (sg == 2) { u = a; v = b; Bt = P.T2N; }
+    else { u = b; v = c; Bt = P.T2TN; }
+  } else {
+    if (sg == 0) { u = c; v = a; Bt = P.V3; }
+    else if (sg == 1) { u = b; v = a; Bt = P.V3; }
+    else if (sg == 2) { u = a; v = b; Bt = P.T2TN; }
+    else { u = a; v = c; Bt = P.T2TN; }
+  }
+  return Bt + (u + P.Nv * v) * P.No * (sg < 2 ? P.Kv : P.Ko);
 }
 
 template <int FM, int FN>
@@ -75,8 +101,7 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
   const int wm = wave >> 1, wn = wave & 1;
   const int64_t No = P.No, N2 = P.N2, Nv = P.Nv;
 
-  // column information
-  int64_t colB[FN];   // tuple pair indices are resolved per segment below
+  // columns (t, rr) of this lane's B fragments
   int64_t ca[FN], cb[FN], cc[FN], cslot[FN], crr[FN];
   bool cvalid[FN];
   bool any = false;
@@ -94,23 +119,21 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
     }
     cvalid[g] = ok;
     any |= ok;
-    // clamp to a valid address for the loads of invalid columns
-    if (!ok) { a = 0; b = 0; c = 0; crr[g] = 0; }
+    if (!ok) { a = 0; b = 0; c = 0; crr[g] = 0; }  // safe load address
     ca[g] = a; cb[g] = b; cc[g] = c;
     cslot[g] = slot;
-    colB[g] = 0;
   }
   if (!__syncthreads_or(any)) return;
 
-  // A-pair indices of this class (valid whenever a column is valid)
-  int64_t x1, x2;  // class w: (a,b)  v: (a,c)  u: (b,c)
+  // A-pair of this class: w (a,b), v (a,c), u (b,c)
+  int64_t x1, x2;
   if (cl == 0) { x1 = al * nb + o1; x2 = be * nb + o2; }
   else if (cl == 1) { x1 = al * nb + o1; x2 = ga * nb + o2; }
   else { x1 = be * nb + o1; x2 = ga * nb + o2; }
   if (x1 >= Nv) x1 = Nv - 1;
   if (x2 >= Nv) x2 = Nv - 1;
 
-  // rows
+  // rows: straight (p + No q) and swapped (q + No p) row of the A slab
   int64_t rowS[FM], rowW[FM];
 #pragma unroll
   for (int f = 0; f < FM; f++) {
@@ -121,89 +144,74 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
     rowW[f] = q + No * p;
   }
 
+  const int64_t Kv = P.Kv, Ko = P.Ko;
+  // segments: 0 T3[x1] straight, 1 T3[x2] swapped, 2 H3[x2] straight,
+  //           3 H3[x1] swapped
+#define SET_PTRS(SG)                                                        \
+  {                                                                         \
+    const int64_t K_ = (SG) < 2 ? Kv : Ko;                                  \
+    const double *Ab_ = (SG) < 2 ? P.T3 + ((SG) == 0 ? x1 : x2) * N2 * Kv   \
+                                 : P.H3 + ((SG) == 2 ? x2 : x1) * N2 * Ko;  \
+    _Pragma("unroll") for (int f = 0; f < FM; f++) ap[f] =                  \
+        Ab_ + (((SG)&1) ? rowW[f] : rowS[f]) * K_ + 4 * kk;                 \
+    _Pragma("unroll") for (int g = 0; g < FN; g++) bp[g] =                  \
+        b_base(P, cl, (SG), ca[g], cb[g], cc[g]) + crr[g] * K_ + 4 * kk;    \
+  }
   d4 acc[FM][FN];
 #pragma unroll
   for (int f = 0; f < FM; f++)
 #pragma unroll
     for (int g = 0; g < FN; g++) acc[f][g] = d4{0.0, 0.0, 0.0, 0.0};
 
-  const int64_t Kv = P.Kv, Ko = P.Ko;
+  // Software pipeline: two operand sets in ping-pong, so the loads of one
+  // chunk are in flight while the MFMAs of the other run (no register
+  // copies between the sets, hence no wait on the in-flight loads).  The
+  // four K segments run one after the other.
+  const double *ap[FM], *bp[FN];
+#define LOAD(AV, BV, KOFF)                                              \
+  {                                                                     \
+    _Pragma("unroll") for (int f = 0; f < FM; f++) AV[f] =              \
+        *(const d4 *)(ap[f] + (KOFF));                                  \
+    _Pragma("unroll") for (int g = 0; g < FN; g++) BV[g] =              \
+        *(const d4 *)(bp[g] + (KOFF));                                  \
+    __builtin_amdgcn_sched_barrier(0);                                  \
+  }
+#define COMPUTE(AV, BV)                                                 \
+  {                                                                     \
+    _Pragma("unroll") for (int s = 0; s < 4; s++)                       \
+        _Pragma("unroll") for (int f = 0; f < FM; f++)                  \
+            _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =  \
+                __builtin_amdgcn_mfma_f64_16x16x4f64(AV[f][s], BV[g][s], \
+                                                     acc[f][g], 0, 0, 0); \
+  }
+  d4 a0[FM], b0[FN], a1[FM], b1[FN];
+#pragma unroll 1
   for (int sg = 0; sg < 4; sg++) {
-    // segment table (see file comment)
-    const double *Abase;
-    bool swap;
-    int64_t K;
-    if (sg < 2) {
-      K = Kv;
-      const int64_t xa = (cl == 0) ? (sg == 0 ? x1 : x2)
-                       : (cl == 1) ? (sg == 0 ? x1 : x2)
-                                   : (sg == 0 ? x1 : x2);
-      Abase = P.T3 + xa * N2 * Kv;
-      swap = sg == 1;
-    } else {
-      K = Ko;
-      // w: H5 (b, straight), H2 (a, swapped); v: H1 (c), H4 (a, swapped);
-      // u: H0 (c), H3 (b, swapped)
-      int64_t xa;
-      if (cl == 0) xa = (sg == 2) ? x2 : x1;
-      else if (cl == 1) xa = (sg == 2) ? x2 : x1;
-      else xa = (sg == 2) ? x2 : x1;
-      Abase = P.H3 + xa * N2 * Ko;
-      swap = sg == 3;
-    }
-    const double *ap[FM];
-#pragma unroll
-    for (int f = 0; f < FM; f++)
-      ap[f] = Abase + (swap ? rowW[f] : rowS[f]) * K + 4 * kk;
-    const double *bp[FN];
-#pragma unroll
-    for (int g = 0; g < FN; g++) {
-      const int64_t a = ca[g], b = cb[g], c = cc[g];
-      int64_t u, v;  // B pair (u,v) -> index u + Nv v
-      const double *Bt;
-      if (cl == 0) {
-        if (sg == 0) { u = b; v = c; Bt = P.V3; }
-        else if (sg == 1) { u = a; v = c; Bt = P.V3; }
-        else if (sg == 2) { u = a; v = c; Bt = P.T2N; }
-        else { u = b; v = c; Bt = P.T2N; }
-      } else if (cl == 1) {
-        if (sg == 0) { u = c; v = b; Bt = P.V3; }
-        else if (sg == 1) { u = a; v = b; Bt = P.V3; }
-        else if (sg == 2) { u = a; v = b; Bt = P.T2N; }
-        else { u = b; v = c; Bt = P.T2TN; }
-      } else {
-        if (sg == 0) { u = c; v = a; Bt = P.V3; }
-        else if (sg == 1) { u = b; v = a; Bt = P.V3; }
-        else if (sg == 2) { u = a; v = b; Bt = P.T2TN; }
-        else { u = a; v = c; Bt = P.T2TN; }
-      }
-      bp[g] = Bt + ((u + Nv * v) * No + crr[g]) * K + 4 * kk;
-    }
-    for (int64_t k0 = 0; k0 < K; k0 += KCHUNK) {
-      d4 av[FM], bv[FN];
-#pragma unroll
-      for (int f = 0; f < FM; f++) av[f] = *(const d4 *)(ap[f] + k0);
-#pragma unroll
-      for (int g = 0; g < FN; g++) bv[g] = *(const d4 *)(bp[g] + k0);
-#pragma unroll
-      for (int s = 0; s < 4; s++)
-#pragma unroll
-        for (int f = 0; f < FM; f++)
-#pragma unroll
-          for (int g = 0; g < FN; g++)
-            acc[f][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                av[f][s], bv[g][s], acc[f][g], 0, 0, 0);
+    SET_PTRS(sg);
+    // the prefetch is issued unconditionally (clamped to the last chunk) so
+    // every path into a COMPUTE has the same loads outstanding
+    const int64_t last = (sg < 2 ? Kv : Ko) - KCHUNK;
+    LOAD(a0, b0, 0);
+    for (int64_t k0 = 0;; k0 += 2 * KCHUNK) {
+      LOAD(a1, b1, k0 + KCHUNK < last ? k0 + KCHUNK : last);
+      COMPUTE(a0, b0);
+      if (k0 + KCHUNK > last) break;
+      LOAD(a0, b0, k0 + 2 * KCHUNK < last ? k0 + 2 * KCHUNK : last);
+      COMPUTE(a1, b1);
+      if (k0 + 2 * KCHUNK > last) break;
     }
   }
-  (void)colB;
+#undef LOAD
+#undef COMPUTE
 
+#undef SET_PTRS
   // store: for v_mfma_f64_16x16x4f64, D[i][j] sits in lane j + 16*(i%4),
   // register i/4 (measured on gfx950)
-  double *Gc = P.G + ((int64_t)cl * P.slots + w * (int64_t)nb * nb * nb) * P.N3;
+  double *Gc = P.G + ((int64_t)cl * P.slots + w * (int64_t)nb * nb * nb) * P.W3;
 #pragma unroll
   for (int g = 0; g < FN; g++) {
     if (!cvalid[g]) continue;
-    double *Gs = Gc + cslot[g] * P.N3;
+    double *Gs = Gc + cslot[g] * P.W3;
     const int64_t rr = crr[g];
 #pragma unroll
     for (int f = 0; f < FM; f++) {
@@ -213,11 +221,11 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
             (int64_t)mt * MT + wm * (MT / 2) + f * 16 + 4 * r + kk;
         if (m >= N2) continue;
         const int64_t p = m % No, q = m / No;
-        int64_t addr;
-        if (cl == 0) addr = m + N2 * rr;            // (i,j,k) = (p,q,rr)
-        else if (cl == 1) addr = p + No * rr + N2 * q;  // (p,rr,q)
-        else addr = rr + No * m;                    // (rr,p,q)
-        Gs[addr] = acc[f][g][r];
+        int64_t i, j, k;
+        if (cl == 0) { i = p; j = q; k = rr; }        // "ij|k"
+        else if (cl == 1) { i = p; j = rr; k = q; }   // "ik|j"
+        else { i = rr; j = p; k = q; }                // "jk|i"
+        Gs[w_index(i, j, k, P.nbk)] = acc[f][g][r];
       }
     }
   }
@@ -242,14 +250,36 @@ hipError_t launch_doubles_f64(const DevSystem &s, const BatchArgs &b,
   P.Kv = s.Kv;
   P.Ko = s.Ko;
   P.N2 = s.No * s.No;
-  P.N3 = P.N2 * s.No;
+  P.nbk = w_pad(s.No) / WCUBE;
+  P.W3 = w_pad(s.No) * w_pad(s.No) * w_pad(s.No);
   P.slots = b.slots;
   P.first = b.first;
   P.last = b.last;
   P.nb = b.nb;
-  int FM = 2, FN = 2;
-  if (tile == 1) { FM = 4; FN = 2; }
-  if (tile == 2) { FM = 2; FN = 4; }
+  // tile variants (wave tile FM x FN fragments of 16x16, 2x2 waves), with
+  // the fraction of FP64 peak each reached on MI355X at No=40, Nv=400
+  static const int tiles[][2] = {{2, 2}, {4, 2}, {2, 4}, {4, 4},
+                                 {3, 3}, {5, 5}, {4, 5}, {5, 4}};
+  static const double measured[] = {0.44, 0.56, 0.51, 0.57,
+                                    0.53, 0.69, 0.60, 0.60};
+  const int nt = (int)(sizeof(tiles) / sizeof(tiles[0]));
+  if (tile < 0 || tile >= nt) {
+    // automatic: best measured rate times the useful fraction of the
+    // padded M = No^2 by N = nb*No tile grid
+    double best = -1.0;
+    const double M = (double)P.N2, N = (double)b.nb * s.No;
+    for (int t = 0; t < nt; t++) {
+      const double mt = 32.0 * tiles[t][0], ntl = 32.0 * tiles[t][1];
+      const double use = M / (std::ceil(M / mt) * mt) * N / (std::ceil(N / ntl) * ntl);
+      const double score = measured[t] * use;
+      if (score > best) {
+        best = score;
+        tile = t;
+      }
+    }
+  }
+  const int FM = tiles[tile][0];
+  const int FN = tiles[tile][1];
   const int64_t MT = 32 * FM, NT = 32 * FN;
   P.mtiles = (int)((P.N2 + MT - 1) / MT);
   P.ntiles = (int)(((int64_t)b.nb * s.No + NT - 1) / NT);
@@ -257,15 +287,13 @@ hipError_t launch_doubles_f64(const DevSystem &s, const BatchArgs &b,
   P.per_xcd = (P.total + 7) / 8;
   const int64_t grid = P.per_xcd * 8;
   if (grid <= 0) return hipSuccess;
-  if (FM == 4)
-    hipLaunchKernelGGL((doubles_f64_kernel<4, 2>), dim3((unsigned)grid),
+#define LAUNCH(A, B)                                                       \
+  if (FM == A && FN == B)                                                  \
+    hipLaunchKernelGGL((doubles_f64_kernel<A, B>), dim3((unsigned)grid),   \
                        dim3(256), 0, st, P);
-  else if (FN == 4)
-    hipLaunchKernelGGL((doubles_f64_kernel<2, 4>), dim3((unsigned)grid),
-                       dim3(256), 0, st, P);
-  else
-    hipLaunchKernelGGL((doubles_f64_kernel<2, 2>), dim3((unsigned)grid),
-                       dim3(256), 0, st, P);
+  LAUNCH(2, 2) LAUNCH(4, 2) LAUNCH(2, 4) LAUNCH(4, 4) LAUNCH(3, 3)
+  LAUNCH(5, 5) LAUNCH(4, 5) LAUNCH(5, 4)
+#undef LAUNCH
   return hipGetLastError();
 }
 

@@ -1,8 +1,8 @@
 // Fused singles + energy epilogue of the (T) hot path.
 //
 // One workgroup per tuple slot of a window batch.  W is the sum of the three
-// class buffers written by the doubles kernel, Z = W + singles is formed in
-// registers (reference src/atrip/Equations.cxx:387-426, with Z copied from W
+// class buffers written by the doubles kernel (cube-blocked layout, see
+// w_index), Z = W + singles is formed in registers (reference src/atrip/Equations.cxx:387-426, with Z copied from W
 // as in src/atrip/Atrip.cxx:899-906), and the energy of the tuple follows
 // get_energy_distinct / get_energy_same (src/atrip/Equations.cxx:101-238)
 // selected as src/atrip/Atrip.cxx:636-670 does.  The i>=j>=k points are
@@ -17,11 +17,10 @@ namespace {
 struct EnergyParams {
   const double *G, *GJ;
   const double *Tai, *VHH, *ei, *ea;
-  const uint32_t *points;
   const int32_t *windows;
   const uint64_t *aprefix;
   double *e_slot;
-  int64_t No, Nv, N2, N3, npoints, slots;
+  int64_t No, Nv, N2, W3, nbk, slots;
   uint64_t first, last;
   int nb;
   double tai_sign;
@@ -34,7 +33,14 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-__global__ __launch_bounds__(256) void energy_kernel(EnergyParams P) {
+// Energy of one tuple per workgroup.  The i>=j>=k points are visited per
+// block triple (B1>=B2>=B3) of edge 8: the (up to) six cubes that hold W at
+// the six index permutations of those points are summed from the three class
+// buffers into LDS with coalesced 4 KiB reads, then every point reads its
+// twelve W/Z values from LDS.
+template <bool HJ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void energy_kernel(EnergyParams P) {
   const int64_t gslot = blockIdx.x;
   const int nb = P.nb;
   const int64_t nb3 = (int64_t)nb * nb * nb;
@@ -56,113 +62,174 @@ __global__ __launch_bounds__(256) void energy_kernel(EnergyParams P) {
     }
     return;
   }
+  constexpr bool have_j = HJ;
 
-  extern __shared__ double sm[];
-  double *tA = sm, *tB = sm + No, *tC = sm + 2 * No;
-  for (int64_t q = threadIdx.x; q < No; q += blockDim.x) {
-    tA[q] = P.tai_sign * P.Tai[a + Nv * q];
-    tB[q] = P.tai_sign * P.Tai[b + Nv * q];
-    tC[q] = P.tai_sign * P.Tai[c + Nv * q];
-  }
-  // V(xy)ij slabs of the singles term stay in global memory (L1/L2 hits)
+  __shared__ double cube[6][512];
+  __shared__ double cubeJ[HJ ? 6 : 1][HJ ? 512 : 1];
+  __shared__ double tA[64], tB[64], tC[64];  // No <= 64 handled in tiles below
+  __shared__ double red[2][4];
+
   const double *Vbc = P.VHH + (b + Nv * c) * N2;
   const double *Vac = P.VHH + (a + Nv * c) * N2;
   const double *Vab = P.VHH + (a + Nv * b) * N2;
-  __syncthreads();
-
-  const double *Gw = P.G + (0 * P.slots + gslot) * P.N3;
-  const double *Gv = P.G + (1 * P.slots + gslot) * P.N3;
-  const double *Gu = P.G + (2 * P.slots + gslot) * P.N3;
-  const bool have_j = P.GJ != nullptr;
-  const double *Jw = have_j ? P.GJ + (0 * P.slots + gslot) * P.N3 : Gw;
-  const double *Jv = have_j ? P.GJ + (1 * P.slots + gslot) * P.N3 : Gv;
-  const double *Ju = have_j ? P.GJ + (2 * P.slots + gslot) * P.N3 : Gu;
+  const double *Gw = P.G + (0 * P.slots + gslot) * P.W3;
+  const double *Gv = P.G + (1 * P.slots + gslot) * P.W3;
+  const double *Gu = P.G + (2 * P.slots + gslot) * P.W3;
+  const double *Jw = have_j ? P.GJ + (0 * P.slots + gslot) * P.W3 : Gw;
+  const double *Jv = have_j ? P.GJ + (1 * P.slots + gslot) * P.W3 : Gv;
+  const double *Ju = have_j ? P.GJ + (2 * P.slots + gslot) * P.W3 : Gu;
 
   int distinct = 0;
   if (a == b) distinct++;
   if (b == c) distinct--;
   const double epsabc = (P.ea[a] + P.ea[b]) + P.ea[c];
-
-  auto Wat = [&](int64_t x, int64_t y, int64_t z) {
-    const int64_t q = x + No * y + N2 * z;
-    return (Gw[q] + Gv[q]) + Gu[q];
-  };
-  auto WJat = [&](int64_t x, int64_t y, int64_t z) {
-    const int64_t q = x + No * y + N2 * z;
-    return (Jw[q] + Jv[q]) + Ju[q];
-  };
-  auto Zat = [&](double wv, int64_t x, int64_t y, int64_t z) {
-    double zz = wv + tA[x] * Vbc[y + No * z];
-    zz = zz + tB[y] * Vac[x + No * z];
-    zz = zz + tC[z] * Vab[x + No * y];
-    return zz;
-  };
+  const int nbk = (int)P.nbk;
+  const int no = (int)No;
+  const int tid = threadIdx.x;
 
   double e_acc = 0.0, ect_acc = 0.0;
-  for (int64_t pt = threadIdx.x; pt < P.npoints; pt += blockDim.x) {
-    const uint32_t packed = P.points[pt];
-    const int64_t i = packed & 1023u, j = (packed >> 10) & 1023u,
-                  k = (packed >> 20) & 1023u;
-    const double ei = P.ei[i], ej = P.ei[j], ek = P.ei[k];
-    const double facjk = j == k ? 0.5 : 1.0;
-    const double facij = i == j ? 0.5 : 1.0;
-    const double eijk = (ei + ej) + ek;
-    const double denominator = epsabc - eijk;
-    if (distinct == 0) {
-      const double A = Wat(i, j, k), B = Wat(i, k, j), C = Wat(j, i, k),
-                   D = Wat(j, k, i), E = Wat(k, i, j), F = Wat(k, j, i);
-      const double U = Zat(A, i, j, k), V = Zat(B, i, k, j),
-                   W = Zat(C, j, i, k), X = Zat(D, j, k, i),
-                   Y = Zat(E, k, i, j), Z = Zat(F, k, j, i);
-      const double UXY = U + (X + Y), VWZ = V + (W + Z);
-      {
-        const double first =
-            A * U + (B * V + (C * W + (D * X + (E * Y + F * Z))));
-        const double second = (UXY - 2.0 * VWZ) * (A + (D + E));
-        const double third = (VWZ - 2.0 * UXY) * (B + (C + F));
-        const double value = 3.0 * first + (second + third);
-        e_acc += ((2.0 * value) / denominator) * (facjk * facij);
+#pragma unroll 1
+  for (int B3 = 0; B3 < nbk; B3++)
+#pragma unroll 1
+    for (int B2 = B3; B2 < nbk; B2++)
+#pragma unroll 1
+      for (int B1 = B2; B1 < nbk; B1++) {
+        // cube coordinates of the permutations ijk ikj jik jki kij kji
+        const int cx[6] = {B1, B1, B2, B2, B3, B3};
+        const int cy[6] = {B2, B3, B1, B3, B1, B2};
+        const int cz[6] = {B3, B2, B3, B1, B2, B1};
+        int alias[6];
+#pragma unroll
+        for (int p = 0; p < 6; p++) {
+          alias[p] = p;
+          for (int q = 0; q < p; q++)
+            if (alias[p] == p && cx[q] == cx[p] && cy[q] == cy[p] &&
+                cz[q] == cz[p])
+              alias[p] = q;
+        }
+        __syncthreads();  // previous triple done with the LDS cubes
+        // singles tables for the three blocks (indices i,j,k < No)
+        if (tid < 8) {
+          const int i = B1 * 8 + tid, j = B2 * 8 + tid, k = B3 * 8 + tid;
+          // tA/tB/tC hold t at (a,x),(b,x),(c,x) for x in each block
+          tA[tid] = i < No ? P.tai_sign * P.Tai[a + Nv * i] : 0.0;
+          tA[8 + tid] = j < No ? P.tai_sign * P.Tai[a + Nv * j] : 0.0;
+          tA[16 + tid] = k < No ? P.tai_sign * P.Tai[a + Nv * k] : 0.0;
+          tB[tid] = i < No ? P.tai_sign * P.Tai[b + Nv * i] : 0.0;
+          tB[8 + tid] = j < No ? P.tai_sign * P.Tai[b + Nv * j] : 0.0;
+          tB[16 + tid] = k < No ? P.tai_sign * P.Tai[b + Nv * k] : 0.0;
+          tC[tid] = i < No ? P.tai_sign * P.Tai[c + Nv * i] : 0.0;
+          tC[8 + tid] = j < No ? P.tai_sign * P.Tai[c + Nv * j] : 0.0;
+          tC[16 + tid] = k < No ? P.tai_sign * P.Tai[c + Nv * k] : 0.0;
+        }
+        // distinct cubes of this orbit, loaded as one flat coalesced sweep
+        int cbase[6];
+        int nc = 0, slotof[6];
+#pragma unroll
+        for (int p = 0; p < 6; p++) {
+          slotof[p] = p;
+          if (alias[p] == p) {
+            cbase[nc] = ((cz[p] * nbk + cy[p]) * nbk + cx[p]) << 9;
+            slotof[p] = nc++;
+          }
+        }
+#pragma unroll
+        for (int p = 0; p < 6; p++) alias[p] = slotof[alias[p]];
+#pragma unroll 1
+        for (int e = tid; e < nc * 512; e += 256) {
+          const int p = e >> 9, q = e & 511;
+          int base = cbase[0];
+          if (p == 1) base = cbase[1];
+          else if (p == 2) base = cbase[2];
+          else if (p == 3) base = cbase[3];
+          else if (p == 4) base = cbase[4];
+          else if (p == 5) base = cbase[5];
+          cube[p][q] = (Gw[base + q] + Gv[base + q]) + Gu[base + q];
+          if constexpr (HJ)
+            cubeJ[p][q] = (Jw[base + q] + Jv[base + q]) + Ju[base + q];
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int pt = tid; pt < 512; pt += 256) {
+          const int li = pt & 7, lj = (pt >> 3) & 7, lk = pt >> 6;
+          const int i = B1 * 8 + li, j = B2 * 8 + lj, k = B3 * 8 + lk;
+          if (i >= no || j >= no || k >= no || i < j || j < k) continue;
+          const double ei = P.ei[i], ej = P.ei[j], ek = P.ei[k];
+          const double facjk = j == k ? 0.5 : 1.0;
+          const double facij = i == j ? 0.5 : 1.0;
+          const double eijk = (ei + ej) + ek;
+          const double denominator = epsabc - eijk;
+          // local offsets of the permutations inside their cubes
+          const int o_ijk = li + 8 * lj + 64 * lk, o_ikj = li + 8 * lk + 64 * lj,
+                    o_jik = lj + 8 * li + 64 * lk, o_jki = lj + 8 * lk + 64 * li,
+                    o_kij = lk + 8 * li + 64 * lj, o_kji = lk + 8 * lj + 64 * li;
+          // singles at a permutation (x,y,z): t_a[x] V^bc_yz + t_b[y] V^ac_xz
+          // + t_c[z] V^ab_xy; block slots: 0 -> i, 8 -> j, 16 -> k
+          auto Z = [&](double wv, int sx, int sy, int sz, int x, int y, int z) {
+            double zz = wv + tA[sx] * Vbc[y + no * z];
+            zz = zz + tB[sy] * Vac[x + no * z];
+            zz = zz + tC[sz] * Vab[x + no * y];
+            return zz;
+          };
+          const int si = li, sj = 8 + lj, sk = 16 + lk;
+          if (distinct == 0) {
+            const double A = cube[alias[0]][o_ijk], B = cube[alias[1]][o_ikj],
+                         C = cube[alias[2]][o_jik], D = cube[alias[3]][o_jki],
+                         E = cube[alias[4]][o_kij], F = cube[alias[5]][o_kji];
+            const double U = Z(A, si, sj, sk, i, j, k), V = Z(B, si, sk, sj, i, k, j),
+                         W = Z(C, sj, si, sk, j, i, k), X = Z(D, sj, sk, si, j, k, i),
+                         Y = Z(E, sk, si, sj, k, i, j), Zz = Z(F, sk, sj, si, k, j, i);
+            const double UXY = U + (X + Y), VWZ = V + (W + Zz);
+            {
+              const double first =
+                  A * U + (B * V + (C * W + (D * X + (E * Y + F * Zz))));
+              const double second = (UXY - 2.0 * VWZ) * (A + (D + E));
+              const double third = (VWZ - 2.0 * UXY) * (B + (C + F));
+              const double value = 3.0 * first + (second + third);
+              e_acc += ((2.0 * value) / denominator) * (facjk * facij);
+            }
+            if constexpr (HJ) {
+              const double A2 = cubeJ[alias[0]][o_ijk], B2 = cubeJ[alias[1]][o_ikj],
+                           C2 = cubeJ[alias[2]][o_jik], D2 = cubeJ[alias[3]][o_jki],
+                           E2 = cubeJ[alias[4]][o_kij], F2 = cubeJ[alias[5]][o_kji];
+              const double first =
+                  A2 * U + (B2 * V + (C2 * W + (D2 * X + (E2 * Y + F2 * Zz))));
+              const double second = (UXY - 2.0 * VWZ) * (A2 + (D2 + E2));
+              const double third = (VWZ - 2.0 * UXY) * (B2 + (C2 + F2));
+              const double value = 3.0 * first + (second + third);
+              ect_acc += ((2.0 * value) / denominator) * (facjk * facij);
+            }
+          } else {
+            const double A = cube[alias[0]][o_ijk], B = cube[alias[3]][o_jki],
+                         C = cube[alias[4]][o_kij];
+            const double U = Z(A, si, sj, sk, i, j, k), V = Z(B, sj, sk, si, j, k, i),
+                         W = Z(C, sk, si, sj, k, i, j);
+            const double UVW = U + (V + W);
+            {
+              const double value =
+                  3.0 * ((A * U + B * V) + C * W) - (A + (B + C)) * UVW;
+              e_acc += ((2.0 * value) / denominator) * (facjk * facij);
+            }
+            if constexpr (HJ) {
+              const double A2 = cubeJ[alias[0]][o_ijk], B2 = cubeJ[alias[3]][o_jki],
+                           C2 = cubeJ[alias[4]][o_kij];
+              const double value =
+                  3.0 * ((A2 * U + B2 * V) + C2 * W) - (A2 + (B2 + C2)) * UVW;
+              ect_acc += ((2.0 * value) / denominator) * (facjk * facij);
+            }
+          }
+        }
       }
-      if (have_j) {
-        const double A2 = WJat(i, j, k), B2 = WJat(i, k, j),
-                     C2 = WJat(j, i, k), D2 = WJat(j, k, i),
-                     E2 = WJat(k, i, j), F2 = WJat(k, j, i);
-        const double first =
-            A2 * U + (B2 * V + (C2 * W + (D2 * X + (E2 * Y + F2 * Z))));
-        const double second = (UXY - 2.0 * VWZ) * (A2 + (D2 + E2));
-        const double third = (VWZ - 2.0 * UXY) * (B2 + (C2 + F2));
-        const double value = 3.0 * first + (second + third);
-        ect_acc += ((2.0 * value) / denominator) * (facjk * facij);
-      }
-    } else {
-      const double A = Wat(i, j, k), B = Wat(j, k, i), C = Wat(k, i, j);
-      const double U = Zat(A, i, j, k), V = Zat(B, j, k, i),
-                   W = Zat(C, k, i, j);
-      const double UVW = U + (V + W);
-      {
-        const double value = 3.0 * ((A * U + B * V) + C * W) - (A + (B + C)) * UVW;
-        e_acc += ((2.0 * value) / denominator) * (facjk * facij);
-      }
-      if (have_j) {
-        const double A2 = WJat(i, j, k), B2 = WJat(j, k, i),
-                     C2 = WJat(k, i, j);
-        const double value =
-            3.0 * ((A2 * U + B2 * V) + C2 * W) - (A2 + (B2 + C2)) * UVW;
-        ect_acc += ((2.0 * value) / denominator) * (facjk * facij);
-      }
-    }
-  }
 
-  __shared__ double red[2][4];
   e_acc = wave_sum(e_acc);
   ect_acc = wave_sum(ect_acc);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = tid >> 6, lane = tid & 63;
   if (lane == 0) {
     red[0][wave] = e_acc;
     red[1][wave] = ect_acc;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     const double e = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
     const double ect = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
     P.e_slot[2 * gslot] = e;
@@ -170,24 +237,46 @@ __global__ __launch_bounds__(256) void energy_kernel(EnergyParams P) {
   }
 }
 
-// Neumaier sum of the slot energies of one batch, in slot order.
-__global__ void batch_reduce_kernel(const double *e_slot, int64_t slots,
-                                    double *partials, int index) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Compensated sum of the slot energies of one batch: 256 threads each sum a
+// fixed strided subset (Neumaier), then a fixed pairwise tree of (sum, comp).
+__global__ __launch_bounds__(256) void batch_reduce_kernel(
+    const double *e_slot, int64_t slots, double *partials, int index) {
+  __shared__ double sh[4][256];
   double s0 = 0.0, c0 = 0.0, s1 = 0.0, c1 = 0.0;
-  for (int64_t q = 0; q < slots; q++) {
-    const double x0 = e_slot[2 * q], x1 = e_slot[2 * q + 1];
-    double t = s0 + x0;
-    c0 += (fabs(s0) >= fabs(x0)) ? (s0 - t) + x0 : (x0 - t) + s0;
-    s0 = t;
-    t = s1 + x1;
-    c1 += (fabs(s1) >= fabs(x1)) ? (s1 - t) + x1 : (x1 - t) + s1;
-    s1 = t;
+  auto add = [](double &s, double &c, double x) {
+    const double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+    s = t;
+  };
+  for (int64_t q = threadIdx.x; q < slots; q += 256) {
+    add(s0, c0, e_slot[2 * q]);
+    add(s1, c1, e_slot[2 * q + 1]);
   }
-  partials[4 * index + 0] = s0;
-  partials[4 * index + 1] = c0;
-  partials[4 * index + 2] = s1;
-  partials[4 * index + 3] = c1;
+  sh[0][threadIdx.x] = s0;
+  sh[1][threadIdx.x] = c0;
+  sh[2][threadIdx.x] = s1;
+  sh[3][threadIdx.x] = c1;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      double s = sh[0][threadIdx.x], c = sh[1][threadIdx.x] + sh[1][threadIdx.x + h];
+      add(s, c, sh[0][threadIdx.x + h]);
+      sh[0][threadIdx.x] = s;
+      sh[1][threadIdx.x] = c;
+      s = sh[2][threadIdx.x];
+      c = sh[3][threadIdx.x] + sh[3][threadIdx.x + h];
+      add(s, c, sh[2][threadIdx.x + h]);
+      sh[2][threadIdx.x] = s;
+      sh[3][threadIdx.x] = c;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[4 * index + 0] = sh[0][0];
+    partials[4 * index + 1] = sh[1][0];
+    partials[4 * index + 2] = sh[2][0];
+    partials[4 * index + 3] = sh[3][0];
+  }
 }
 
 }  // namespace
@@ -202,30 +291,32 @@ hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
   P.VHH = s.VHH;
   P.ei = s.ei;
   P.ea = s.ea;
-  P.points = s.points;
   P.windows = b.windows;
   P.aprefix = s.aprefix;
   P.e_slot = e_slot;
   P.No = s.No;
   P.Nv = s.Nv;
   P.N2 = s.No * s.No;
-  P.N3 = P.N2 * s.No;
-  P.npoints = s.npoints;
+  P.nbk = w_pad(s.No) / WCUBE;
+  P.W3 = w_pad(s.No) * w_pad(s.No) * w_pad(s.No);
   P.slots = b.slots;
   P.first = b.first;
   P.last = b.last;
   P.nb = b.nb;
   P.tai_sign = tai_sign;
-  const size_t shm = sizeof(double) * (3 * s.No);
-  hipLaunchKernelGGL(energy_kernel, dim3((unsigned)b.slots), dim3(256), shm,
-                     st, P);
+  if (GJ)
+    hipLaunchKernelGGL(energy_kernel<true>, dim3((unsigned)b.slots), dim3(256),
+                       0, st, P);
+  else
+    hipLaunchKernelGGL(energy_kernel<false>, dim3((unsigned)b.slots),
+                       dim3(256), 0, st, P);
   return hipGetLastError();
 }
 
 hipError_t launch_batch_reduce(const DevSystem &, const BatchArgs &b,
                                const double *e_slot, double *partials,
                                int index, hipStream_t st) {
-  hipLaunchKernelGGL(batch_reduce_kernel, dim3(1), dim3(64), 0, st, e_slot,
+  hipLaunchKernelGGL(batch_reduce_kernel, dim3(1), dim3(256), 0, st, e_slot,
                      b.slots, partials, index);
   return hipGetLastError();
 }

@@ -17,6 +17,22 @@ namespace atrip_amd {
 constexpr int KCHUNK = 16;
 inline int64_t pad16(int64_t n) { return (n + 15) / 16 * 16; }
 
+// Layout of the per-tuple class buffers written by the doubles kernels and
+// read by the energy kernel: W(i,j,k) of one tuple is stored as cubes of edge
+// 8 (512 contiguous doubles each, i fastest inside a cube, cubes ordered
+// i-block fastest), over an occupied range padded to a multiple of 8.  A
+// cube is one coalesced 4 KiB read, which is what the energy kernel's
+// permutation gathers need.
+constexpr int WCUBE = 8;
+__host__ __device__ inline int64_t w_pad(int64_t No) {
+  return (No + WCUBE - 1) / WCUBE * WCUBE;
+}
+__host__ __device__ inline int64_t w_index(int64_t i, int64_t j, int64_t k,
+                                           int64_t nbk) {
+  return ((((k >> 3) * nbk + (j >> 3)) * nbk + (i >> 3)) << 9) + (i & 7) +
+         ((j & 7) << 3) + ((k & 7) << 6);
+}
+
 // Device-resident system.  Pair-indexed arrays use the CTF pair order
 // p(x,y) = x + Nv*y.  Layouts (all row-major with the last index fastest):
 //   T3  [x][p + No q][E < Kv]    = Tpphh[x,E,p,q]          (T(x) slices)
@@ -70,7 +86,7 @@ struct atrip_hip_ctx {
   int window = 8;
   int batch = 4;
   int timing = 0;
-  int tile = 0;
+  int tile = -1;  // -1: chosen per system shape
   // scratch, grown on demand
   double *G = nullptr;       // class buffers [ncls][slots][No^3]
   size_t G_bytes = 0;

@@ -110,7 +110,7 @@ def main(argv=None):
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split_fp32"])
     ap.add_argument("--window", type=int, default=8)
     ap.add_argument("--batch", type=int, default=4)
-    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--slices", type=int, default=16,
                     help="steps that cover the tuple space once")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

@@ -101,7 +101,7 @@ int atrip_hip_tuple_energies(atrip_hip_ctx *ctx, int sys, int precision,
  *   "window"  block edge nb of the tuple windows (1..32)
  *   "batch"   windows per launch
  *   "timing"  1: time every hot-kernel launch with HIP events
- *   "tile"    doubles-kernel tile variant (see DESIGN notes in the source) */
+ *   "tile"    doubles-kernel tile variant 0..7, -1 (default) = automatic */
 int atrip_hip_set_option(atrip_hip_ctx *ctx, const char *key, int64_t value);
 int atrip_hip_get_option(atrip_hip_ctx *ctx, const char *key, int64_t *value);
 

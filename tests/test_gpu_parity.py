@@ -118,12 +118,13 @@ def test_partition_independence(hip_ctx, oracle):
     try:
         n = oracle.n_tuples(nv)
         outs = []
-        for window, batch, tile in [(8, 4, 0), (4, 1, 0), (16, 2, 1), (5, 7, 2)]:
+        for window, batch, tile in [(8, 4, 0), (4, 1, 0), (16, 2, 1), (5, 7, 2), (8, 2, 5),
+                                    (3, 5, 3), (8, 1, -1)]:
             hip_ctx.set_option("window", window)
             hip_ctx.set_option("batch", batch)
             hip_ctx.set_option("tile", tile)
             outs.append(hip_ctx.tuple_energies(s, 0, n))
-        hip_ctx.set_option("tile", 0)
+        hip_ctx.set_option("tile", -1)
         hip_ctx.set_option("window", 8)
         hip_ctx.set_option("batch", 4)
         # a single tuple alone gives the same bits as in the full run
