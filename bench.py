@@ -64,6 +64,23 @@ def step_slices(nv: int, nb: int, nsteps: int):
     return out
 
 
+def measured_traffic(kernel: str, no: int, nv: int):
+    """HBM bytes per launch of `kernel` at this shape, as measured with
+    rocprofv3 FETCH_SIZE + WRITE_SIZE runs and recorded in profiles/ (None
+    when no such measurement exists)."""
+    import glob
+    import re
+
+    key = f"roofline_traffic_bytes_per_launch {kernel} No={no} Nv={nv}:"
+    for path in sorted(glob.glob(os.path.join(HERE, "profiles", "*.md"))):
+        for line in open(path):
+            if line.startswith(key):
+                m = re.search(r":\s*([0-9.eE+]+)", line[len(key) - 1:])
+                if m:
+                    return float(m.group(1))
+    return None
+
+
 def cpu_baseline(no: int, nv: int, seed: int, budget_s: float, cores: int):
     """Oracle (C port of the reference np=1 CPU path, host BLAS dgemm) on the
     first tuples of the same workload, all host cores as tuple shards."""
@@ -195,7 +212,9 @@ def main(argv=None):
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": None,
+            "traffic": measured_traffic(
+                "doubles_f64" if args.precision == "fp64" else "doubles_split",
+                args.no, args.nv),
             "kernel": kname,
             "launches": ks.launches,
             "avg_launch_ms": ks.avg_ms,
