@@ -50,6 +50,37 @@ double *dev_alloc(DevSystem &s, int64_t n, const char *what) {
   return (double *)p;
 }
 
+void *dev_bytes(DevSystem &s, size_t bytes, const char *what) {
+  void *p = nullptr;
+  check(hipMalloc(&p, std::max<size_t>(bytes, 1)), what);
+  s.allocations.push_back(p);
+  s.bytes += bytes;
+  return p;
+}
+
+// Split-FP32 copies of the GEMM operands, made once per system on the first
+// mixed-precision run.
+void ensure_split(atrip_hip_ctx *ctx, DevSystem &s) {
+  if (s.split_ready) return;
+  const int64_t No = s.No, Nv = s.Nv, N2 = No * No, Nv2 = Nv * Nv;
+  auto make = [&](const double *x, DevSystem::Split &o, int64_t n) {
+    if (!x) return;
+    o.h = (float *)dev_bytes(s, (size_t)n * sizeof(float), "alloc split");
+    o.b = (uint16_t *)dev_bytes(s, (size_t)n * 2, "alloc split");
+    o.l = (uint16_t *)dev_bytes(s, (size_t)n * 2, "alloc split");
+    check(launch_split(x, o, n, ctx->stream), "split conversion");
+  };
+  make(s.T3, s.sT3, Nv * N2 * s.Kv);
+  make(s.V3, s.sV3, Nv2 * No * s.Kv);
+  make(s.H3, s.sH3, Nv * N2 * s.Ko);
+  make(s.T2N, s.sT2N, Nv2 * No * s.Ko);
+  make(s.T2TN, s.sT2TN, Nv2 * No * s.Ko);
+  make(s.V3J, s.sV3J, Nv2 * No * s.Kv);
+  make(s.H3J, s.sH3J, Nv * N2 * s.Ko);
+  check(hipStreamSynchronize(ctx->stream), "split conversion");
+  s.split_ready = true;
+}
+
 void free_system(DevSystem &s) {
   for (void *p : s.allocations) (void)hipFree(p);
   s.allocations.clear();
@@ -282,6 +313,11 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   const uint64_t n = n_tuples(s.Nv);
   last = std::min(last, n);
   RunResult res;
+  if (precision != ATRIP_FP64 && precision != ATRIP_SPLIT_FP32 &&
+      precision != ATRIP_FP32)
+    throw Fail{"unknown precision " + std::to_string(precision)};
+  const bool g32 = precision != ATRIP_FP64;
+  if (g32) ensure_split(ctx, s);
   if (first >= last) return res;
   const int nb = plan.nb;
   const bool full = first == 0 && last == n;
@@ -330,17 +366,20 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   const int64_t slots_max = (int64_t)std::min(bw, nsel) * nb3;
   const int64_t W3 = w_pad(s.No) * w_pad(s.No) * w_pad(s.No);
   const bool use_j = s.V3J != nullptr;
-  const size_t gsz = (size_t)(3 * slots_max * W3) * sizeof(double);
+  const size_t gsz =
+      (size_t)(3 * slots_max * W3) * (g32 ? sizeof(float) : sizeof(double));
   ensure((void *&)ctx->G, ctx->G_bytes, use_j ? 2 * gsz : gsz, "alloc G");
   ensure((void *&)ctx->e_slot, ctx->e_slot_bytes,
          (size_t)(2 * slots_max) * sizeof(double), "alloc e_slot");
   const int nbatch = (nsel + bw - 1) / bw;
   ensure((void *&)ctx->partials, ctx->partials_bytes,
          (size_t)(4 * nbatch) * sizeof(double), "alloc partials");
-  double *G = ctx->G;
-  double *GJ = use_j ? ctx->G + 3 * slots_max * W3 : nullptr;
+  void *G = ctx->G;
+  void *GJ = use_j ? (void *)((char *)ctx->G + gsz) : nullptr;
   const double flop_tuple = 12.0 * (double)s.No * s.No * s.No * (s.No + s.Nv);
-  const char *dname = precision == ATRIP_FP64 ? "doubles" : "doubles_mixed";
+  const char *dname = precision == ATRIP_FP64        ? "doubles"
+                      : precision == ATRIP_SPLIT_FP32 ? "doubles_mixed"
+                                                      : "doubles_fp32";
   std::vector<double> host_slots;
 
   for (int bi = 0; bi < nbatch; bi++) {
@@ -362,18 +401,19 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
       check(hipEventRecord(e0, ctx->stream), "event");
     }
     for (int pass = 0; pass < (use_j ? 2 : 1); pass++) {
-      double *Gp = pass == 0 ? G : GJ;
+      void *Gp = pass == 0 ? G : GJ;
       if (precision == ATRIP_FP64)
-        check(launch_doubles_f64(s, b, pass == 1, Gp, ctx->tile, ctx->stream),
+        check(launch_doubles_f64(s, b, pass == 1, (double *)Gp, ctx->tile,
+                                 ctx->stream),
               "doubles kernel");
-      else if (precision == ATRIP_SPLIT_FP32)
-        check(launch_doubles_split(s, b, pass == 1, Gp, ctx->stream),
-              "doubles (split fp32) kernel");
       else
-        throw Fail{"unknown precision"};
+        check(launch_doubles_split(s, b, pass == 1, (float *)Gp,
+                                   precision == ATRIP_SPLIT_FP32, ctx->tile,
+                                   ctx->stream),
+              "doubles (FP32 products) kernel");
     }
     if (ctx->timing) check(hipEventRecord(e1, ctx->stream), "event");
-    check(launch_energy(s, b, G, GJ, ijkabc ? -1.0 : 1.0, ctx->e_slot,
+    check(launch_energy(s, b, G, GJ, g32, ijkabc ? -1.0 : 1.0, ctx->e_slot,
                         ctx->stream),
           "energy kernel");
     if (ctx->timing) {

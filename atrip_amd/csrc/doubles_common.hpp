@@ -1,0 +1,106 @@
+// Work decomposition shared by the doubles kernels (FP64 and split FP32):
+// the class/segment table of the regrouped doubles contractions.
+#pragma once
+
+#include "internal.hpp"
+
+namespace atrip_amd {
+
+// Column tuple of class `cl` for A-pair offsets (o1,o2) and third offset t;
+// slot is the tuple's position ((ta*nb + tb)*nb + tc) inside its window.
+__device__ inline void class_tuple(int cl, int64_t al, int64_t be, int64_t ga,
+                                   int nb, int o1, int o2, int t, int64_t &a,
+                                   int64_t &b, int64_t &c, int64_t &slot) {
+  if (cl == 0) {
+    a = al * nb + o1; b = be * nb + o2; c = ga * nb + t;
+    slot = ((int64_t)o1 * nb + o2) * nb + t;
+  } else if (cl == 1) {
+    a = al * nb + o1; b = be * nb + t; c = ga * nb + o2;
+    slot = ((int64_t)o1 * nb + t) * nb + o2;
+  } else {
+    a = al * nb + t; b = be * nb + o1; c = ga * nb + o2;
+    slot = ((int64_t)t * nb + o1) * nb + o2;
+  }
+}
+
+// B operand of segment sg of class cl for column tuple (a,b,c): returns the
+// array (0: V(xy) slabs, 1: -T(xy)[L,n], 2: -T(xy)[n,L]) and its pair (u,v).
+//   class w: P0 V(bc), P1 V(ac), H5 -TAC[L,k], H2 -TBC[L,k]
+//   class v: P3 V(cb), P5 V(ab), H1 -TAB[L,j], H4 -TBC[j,L]
+//   class u: P4 V(ca), P2 V(ba), H0 -TAB[i,L], H3 -TAC[i,L]
+__device__ inline int b_pair(int cl, int sg, int64_t a, int64_t b, int64_t c,
+                             int64_t &u, int64_t &v) {
+  if (cl == 0) {
+    if (sg == 0) { u = b; v = c; return 0; }
+    if (sg == 1) { u = a; v = c; return 0; }
+    if (sg == 2) { u = a; v = c; return 1; }
+    u = b; v = c; return 1;
+  }
+  if (cl == 1) {
+    if (sg == 0) { u = c; v = b; return 0; }
+    if (sg == 1) { u = a; v = b; return 0; }
+    if (sg == 2) { u = a; v = b; return 1; }
+    u = b; v = c; return 2;
+  }
+  if (sg == 0) { u = c; v = a; return 0; }
+  if (sg == 1) { u = b; v = a; return 0; }
+  if (sg == 2) { u = a; v = b; return 2; }
+  u = a; v = c; return 2;
+}
+
+// A-pair (x1, x2) of class cl: w (a,b), v (a,c), u (b,c).  The A operand of
+// segment 0 is T(x1) straight, 1 T(x2) swapped, 2 Vhhh(x2) straight,
+// 3 Vhhh(x1) swapped.
+__device__ inline void class_pair(int cl, int64_t al, int64_t be, int64_t ga,
+                                  int nb, int o1, int o2, int64_t &x1,
+                                  int64_t &x2) {
+  if (cl == 0) { x1 = al * nb + o1; x2 = be * nb + o2; }
+  else if (cl == 1) { x1 = al * nb + o1; x2 = ga * nb + o2; }
+  else { x1 = be * nb + o1; x2 = ga * nb + o2; }
+}
+
+// W position (i,j,k) of GEMM element (row (p,q), column index rr).
+__device__ inline void class_out(int cl, int64_t p, int64_t q, int64_t rr,
+                                 int64_t &i, int64_t &j, int64_t &k) {
+  if (cl == 0) { i = p; j = q; k = rr; }       // "ij|k"
+  else if (cl == 1) { i = p; j = rr; k = q; }  // "ik|j"
+  else { i = rr; j = p; k = q; }               // "jk|i"
+}
+
+// Tile shapes (wave tile FM x FN fragments of 16x16, 2x2 waves) and the
+// fraction of peak each reached on MI355X at No=40, Nv=400 in FP64.
+constexpr int N_TILES = 8;
+__host__ __device__ inline int tile_fm(int t) {
+  const int v[N_TILES] = {2, 4, 2, 4, 3, 5, 4, 5};
+  return v[t];
+}
+__host__ __device__ inline int tile_fn(int t) {
+  const int v[N_TILES] = {2, 2, 4, 4, 3, 5, 5, 4};
+  return v[t];
+}
+inline double tile_rate(int t) {
+  const double v[N_TILES] = {0.44, 0.56, 0.51, 0.57, 0.53, 0.69, 0.60, 0.60};
+  return v[t];
+}
+
+// Automatic tile: best measured rate times the useful fraction of the padded
+// M = No^2 by N = nb*No tile grid.
+inline int pick_tile(int tile, int64_t No, int nb) {
+  if (tile >= 0 && tile < N_TILES) return tile;
+  double best = -1.0;
+  int pick = 0;
+  const double M = (double)No * No, N = (double)nb * No;
+  for (int t = 0; t < N_TILES; t++) {
+    const double mt = 32.0 * tile_fm(t), nt = 32.0 * tile_fn(t);
+    const double use = M / (__builtin_ceil(M / mt) * mt) *
+                       N / (__builtin_ceil(N / nt) * nt);
+    const double score = tile_rate(t) * use;
+    if (score > best) {
+      best = score;
+      pick = t;
+    }
+  }
+  return pick;
+}
+
+}  // namespace atrip_amd

@@ -19,9 +19,7 @@
 // resident) into registers.  A K chunk of 16 is split as
 // kappa = k0 + 4*(lane/16) + s, s = 0..3, so every lane reads 4 consecutive
 // doubles (32 B) of one row/column per chunk and feeds 4 MFMA k-steps.
-#include <cmath>
-
-#include "internal.hpp"
+#include "doubles_common.hpp"
 
 namespace atrip_amd {
 namespace {
@@ -39,43 +37,12 @@ struct DoublesParams {
   int64_t total, per_xcd;
 };
 
-// Column tuple of class `cl` for A-pair offsets (o1,o2) and third offset t.
-__device__ inline void class_tuple(int cl, int64_t al, int64_t be, int64_t ga,
-                                   int nb, int o1, int o2, int t, int64_t &a,
-                                   int64_t &b, int64_t &c, int64_t &slot) {
-  if (cl == 0) {
-    a = al * nb + o1; b = be * nb + o2; c = ga * nb + t;
-    slot = ((int64_t)o1 * nb + o2) * nb + t;
-  } else if (cl == 1) {
-    a = al * nb + o1; b = be * nb + t; c = ga * nb + o2;
-    slot = ((int64_t)o1 * nb + t) * nb + o2;
-  } else {
-    a = al * nb + t; b = be * nb + o1; c = ga * nb + o2;
-    slot = ((int64_t)t * nb + o1) * nb + o2;
-  }
-}
-
-// B operand of segment sg for a column tuple: table of the file comment.
+// B operand base of segment sg for a column tuple.
 __device__ inline const double *b_base(const DoublesParams &P, int cl, int sg,
                                        int64_t a, int64_t b, int64_t c) {
   int64_t u, v;
-  const double *Bt;
-  if (cl == 0) {
-    if (sg == 0) { u = b; v = c; Bt = P.V3; }
-    else if (sg == 1) { u = a; v = c; Bt = P.V3; }
-    else if (sg == 2) { u = a; v = c; Bt = P.T2N; }
-    else { u = b; v = c; Bt = P.T2N; }
-  } else if (cl == 1) {
-    if (sg == 0) { u = c; v = b; Bt = P.V3; }
-    else if (sg == 1) { u = a; v = b; Bt = P.V3; }
-    else if (sg == 2) { u = a; v = b; Bt = P.T2N; }
-    else { u = b; v = c; Bt = P.T2TN; }
-  } else {
-    if (sg == 0) { u = c; v = a; Bt = P.V3; }
-    else if (sg == 1) { u = b; v = a; Bt = P.V3; }
-    else if (sg == 2) { u = a; v = b; Bt = P.T2TN; }
-    else { u = a; v = c; Bt = P.T2TN; }
-  }
+  const int which = b_pair(cl, sg, a, b, c, u, v);
+  const double *Bt = which == 0 ? P.V3 : which == 1 ? P.T2N : P.T2TN;
   return Bt + (u + P.Nv * v) * P.No * (sg < 2 ? P.Kv : P.Ko);
 }
 
@@ -102,14 +69,14 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
   const int64_t No = P.No, N2 = P.N2, Nv = P.Nv;
 
   // columns (t, rr) of this lane's B fragments
-  int64_t ca[FN], cb[FN], cc[FN], cslot[FN], crr[FN];
+  int32_t ca[FN], cb[FN], cc[FN], cslot[FN], crr[FN];  // 32-bit: live all loop
   bool cvalid[FN];
   bool any = false;
 #pragma unroll
   for (int g = 0; g < FN; g++) {
     const int64_t n = (int64_t)nt * NT + wn * (NT / 2) + g * 16 + l16;
     const int t = (int)(n / No);
-    crr[g] = n % No;
+    crr[g] = (int32_t)(n % No);
     int64_t a, b, c, slot;
     class_tuple(cl, al, be, ga, nb, o1, o2, t, a, b, c, slot);
     bool ok = t < nb && tuple_ok(Nv, a, b, c);
@@ -120,28 +87,26 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
     cvalid[g] = ok;
     any |= ok;
     if (!ok) { a = 0; b = 0; c = 0; crr[g] = 0; }  // safe load address
-    ca[g] = a; cb[g] = b; cc[g] = c;
-    cslot[g] = slot;
+    ca[g] = (int32_t)a; cb[g] = (int32_t)b; cc[g] = (int32_t)c;
+    cslot[g] = (int32_t)slot;
   }
   if (!__syncthreads_or(any)) return;
 
   // A-pair of this class: w (a,b), v (a,c), u (b,c)
   int64_t x1, x2;
-  if (cl == 0) { x1 = al * nb + o1; x2 = be * nb + o2; }
-  else if (cl == 1) { x1 = al * nb + o1; x2 = ga * nb + o2; }
-  else { x1 = be * nb + o1; x2 = ga * nb + o2; }
+  class_pair(cl, al, be, ga, nb, o1, o2, x1, x2);
   if (x1 >= Nv) x1 = Nv - 1;
   if (x2 >= Nv) x2 = Nv - 1;
 
   // rows: straight (p + No q) and swapped (q + No p) row of the A slab
-  int64_t rowS[FM], rowW[FM];
+  int32_t rowS[FM], rowW[FM];  // 32-bit: live all loop
 #pragma unroll
   for (int f = 0; f < FM; f++) {
     int64_t m = (int64_t)mt * MT + wm * (MT / 2) + f * 16 + l16;
     if (m >= N2) m = N2 - 1;
     const int64_t p = m % No, q = m / No;
-    rowS[f] = m;
-    rowW[f] = q + No * p;
+    rowS[f] = (int32_t)m;
+    rowW[f] = (int32_t)(q + No * p);
   }
 
   const int64_t Kv = P.Kv, Ko = P.Ko;
@@ -222,9 +187,7 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
         if (m >= N2) continue;
         const int64_t p = m % No, q = m / No;
         int64_t i, j, k;
-        if (cl == 0) { i = p; j = q; k = rr; }        // "ij|k"
-        else if (cl == 1) { i = p; j = rr; k = q; }   // "ik|j"
-        else { i = rr; j = p; k = q; }                // "jk|i"
+        class_out(cl, p, q, rr, i, j, k);
         Gs[w_index(i, j, k, P.nbk)] = acc[f][g][r];
       }
     }
@@ -256,30 +219,8 @@ hipError_t launch_doubles_f64(const DevSystem &s, const BatchArgs &b,
   P.first = b.first;
   P.last = b.last;
   P.nb = b.nb;
-  // tile variants (wave tile FM x FN fragments of 16x16, 2x2 waves), with
-  // the fraction of FP64 peak each reached on MI355X at No=40, Nv=400
-  static const int tiles[][2] = {{2, 2}, {4, 2}, {2, 4}, {4, 4},
-                                 {3, 3}, {5, 5}, {4, 5}, {5, 4}};
-  static const double measured[] = {0.44, 0.56, 0.51, 0.57,
-                                    0.53, 0.69, 0.60, 0.60};
-  const int nt = (int)(sizeof(tiles) / sizeof(tiles[0]));
-  if (tile < 0 || tile >= nt) {
-    // automatic: best measured rate times the useful fraction of the
-    // padded M = No^2 by N = nb*No tile grid
-    double best = -1.0;
-    const double M = (double)P.N2, N = (double)b.nb * s.No;
-    for (int t = 0; t < nt; t++) {
-      const double mt = 32.0 * tiles[t][0], ntl = 32.0 * tiles[t][1];
-      const double use = M / (std::ceil(M / mt) * mt) * N / (std::ceil(N / ntl) * ntl);
-      const double score = measured[t] * use;
-      if (score > best) {
-        best = score;
-        tile = t;
-      }
-    }
-  }
-  const int FM = tiles[tile][0];
-  const int FN = tiles[tile][1];
+  tile = pick_tile(tile, s.No, b.nb);
+  const int FM = tile_fm(tile), FN = tile_fn(tile);
   const int64_t MT = 32 * FM, NT = 32 * FN;
   P.mtiles = (int)((P.N2 + MT - 1) / MT);
   P.ntiles = (int)(((int64_t)b.nb * s.No + NT - 1) / NT);

@@ -2,7 +2,8 @@
 //
 // One workgroup per tuple slot of a window batch.  W is the sum of the three
 // class buffers written by the doubles kernel (cube-blocked layout, see
-// w_index), Z = W + singles is formed in registers (reference src/atrip/Equations.cxx:387-426, with Z copied from W
+// w_index; FP64, or FP32 for the single-precision-product forms, widened to
+// FP64 on load), Z = W + singles is formed in registers (reference src/atrip/Equations.cxx:387-426, with Z copied from W
 // as in src/atrip/Atrip.cxx:899-906), and the energy of the tuple follows
 // get_energy_distinct / get_energy_same (src/atrip/Equations.cxx:101-238)
 // selected as src/atrip/Atrip.cxx:636-670 does.  The i>=j>=k points are
@@ -15,7 +16,7 @@ namespace atrip_amd {
 namespace {
 
 struct EnergyParams {
-  const double *G, *GJ;
+  const void *G, *GJ;  // class buffers, element type = template GT
   const double *Tai, *VHH, *ei, *ea;
   const int32_t *windows;
   const uint64_t *aprefix;
@@ -38,8 +39,8 @@ __device__ inline double wave_sum(double v) {
 // the six index permutations of those points are summed from the three class
 // buffers into LDS with coalesced 4 KiB reads, then every point reads its
 // twelve W/Z values from LDS.
-template <bool HJ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+template <bool HJ, typename GT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void energy_kernel(EnergyParams P) {
   const int64_t gslot = blockIdx.x;
   const int nb = P.nb;
@@ -72,12 +73,13 @@ void energy_kernel(EnergyParams P) {
   const double *Vbc = P.VHH + (b + Nv * c) * N2;
   const double *Vac = P.VHH + (a + Nv * c) * N2;
   const double *Vab = P.VHH + (a + Nv * b) * N2;
-  const double *Gw = P.G + (0 * P.slots + gslot) * P.W3;
-  const double *Gv = P.G + (1 * P.slots + gslot) * P.W3;
-  const double *Gu = P.G + (2 * P.slots + gslot) * P.W3;
-  const double *Jw = have_j ? P.GJ + (0 * P.slots + gslot) * P.W3 : Gw;
-  const double *Jv = have_j ? P.GJ + (1 * P.slots + gslot) * P.W3 : Gv;
-  const double *Ju = have_j ? P.GJ + (2 * P.slots + gslot) * P.W3 : Gu;
+  const GT *G = (const GT *)P.G, *GJ = (const GT *)P.GJ;
+  const GT *Gw = G + (0 * P.slots + gslot) * P.W3;
+  const GT *Gv = G + (1 * P.slots + gslot) * P.W3;
+  const GT *Gu = G + (2 * P.slots + gslot) * P.W3;
+  const GT *Jw = have_j ? GJ + (0 * P.slots + gslot) * P.W3 : Gw;
+  const GT *Jv = have_j ? GJ + (1 * P.slots + gslot) * P.W3 : Gv;
+  const GT *Ju = have_j ? GJ + (2 * P.slots + gslot) * P.W3 : Gu;
 
   int distinct = 0;
   if (a == b) distinct++;
@@ -135,18 +137,46 @@ void energy_kernel(EnergyParams P) {
         }
 #pragma unroll
         for (int p = 0; p < 6; p++) alias[p] = slotof[alias[p]];
-#pragma unroll 1
-        for (int e = tid; e < nc * 512; e += 256) {
-          const int p = e >> 9, q = e & 511;
-          int base = cbase[0];
-          if (p == 1) base = cbase[1];
-          else if (p == 2) base = cbase[2];
-          else if (p == 3) base = cbase[3];
-          else if (p == 4) base = cbase[4];
-          else if (p == 5) base = cbase[5];
-          cube[p][q] = (Gw[base + q] + Gv[base + q]) + Gu[base + q];
-          if constexpr (HJ)
-            cubeJ[p][q] = (Jw[base + q] + Jv[base + q]) + Ju[base + q];
+        // every load of this thread is issued before the first is used
+        // (2 elements per cube per thread), so the cube fetch costs one
+        // memory latency instead of one per element
+        {
+          GT vw[12], vv[12], vu[12];
+#pragma unroll
+          for (int it = 0; it < 12; it++) {
+            const int p = it >> 1, q = tid + 256 * (it & 1);
+            if (p < nc) {
+              const int base = cbase[p];
+              vw[it] = Gw[base + q];
+              vv[it] = Gv[base + q];
+              vu[it] = Gu[base + q];
+            }
+          }
+#pragma unroll
+          for (int it = 0; it < 12; it++) {
+            const int p = it >> 1, q = tid + 256 * (it & 1);
+            if (p < nc)
+              cube[p][q] = ((double)vw[it] + (double)vv[it]) + (double)vu[it];
+          }
+          if constexpr (HJ) {
+#pragma unroll
+            for (int it = 0; it < 12; it++) {
+              const int p = it >> 1, q = tid + 256 * (it & 1);
+              if (p < nc) {
+                const int base = cbase[p];
+                vw[it] = Jw[base + q];
+                vv[it] = Jv[base + q];
+                vu[it] = Ju[base + q];
+              }
+            }
+#pragma unroll
+            for (int it = 0; it < 12; it++) {
+              const int p = it >> 1, q = tid + 256 * (it & 1);
+              if (p < nc)
+                cubeJ[p][q] =
+                    ((double)vw[it] + (double)vv[it]) + (double)vu[it];
+            }
+          }
         }
         __syncthreads();
 #pragma unroll 1
@@ -282,8 +312,8 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(
 }  // namespace
 
 hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
-                         const double *G, const double *GJ, double tai_sign,
-                         double *e_slot, hipStream_t st) {
+                         const void *G, const void *GJ, bool g32,
+                         double tai_sign, double *e_slot, hipStream_t st) {
   EnergyParams P;
   P.G = G;
   P.GJ = GJ;
@@ -304,12 +334,15 @@ hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
   P.last = b.last;
   P.nb = b.nb;
   P.tai_sign = tai_sign;
-  if (GJ)
-    hipLaunchKernelGGL(energy_kernel<true>, dim3((unsigned)b.slots), dim3(256),
-                       0, st, P);
+  const dim3 grid((unsigned)b.slots), block(256);
+  if (GJ && g32)
+    hipLaunchKernelGGL((energy_kernel<true, float>), grid, block, 0, st, P);
+  else if (GJ)
+    hipLaunchKernelGGL((energy_kernel<true, double>), grid, block, 0, st, P);
+  else if (g32)
+    hipLaunchKernelGGL((energy_kernel<false, float>), grid, block, 0, st, P);
   else
-    hipLaunchKernelGGL(energy_kernel<false>, dim3((unsigned)b.slots),
-                       dim3(256), 0, st, P);
+    hipLaunchKernelGGL((energy_kernel<false, double>), grid, block, 0, st, P);
   return hipGetLastError();
 }
 

@@ -49,6 +49,14 @@ struct DevSystem {
          *T2TN = nullptr, *VHH = nullptr, *Tai = nullptr, *ei = nullptr,
          *ea = nullptr;
   double *V3J = nullptr, *H3J = nullptr;  // (cT) integrals, may be null
+  // Split-FP32 copies of the GEMM operands, made on the first mixed run:
+  // h = (float)x, b = bf16(h), l = bf16(x - h), same layouts as above.
+  struct Split {
+    float *h = nullptr;
+    uint16_t *b = nullptr, *l = nullptr;
+  };
+  Split sT3, sV3, sH3, sT2N, sT2TN, sV3J, sH3J;
+  bool split_ready = false;
   uint32_t *points = nullptr;             // packed (i>=j>=k) triples
   int64_t npoints = 0;
   uint64_t *aprefix = nullptr;            // tuples with first index < a
@@ -119,13 +127,16 @@ hipError_t launch_doubles_f64(const DevSystem &s, const BatchArgs &b,
                               hipStream_t st);
 
 // Doubles contractions in split FP32.
+// Doubles contractions with FP32 products (class buffers in FP32):
+// corrections = true adds the hi*lo + lo*hi terms (split FP32).
 hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
-                                bool use_j, double *G, hipStream_t st);
+                                bool use_j, float *G, bool corrections,
+                                int tile, hipStream_t st);
 
 // Singles + energy epilogue; writes e_slot[2*slot + {0,1}].
 hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
-                         const double *G, const double *GJ, double tai_sign,
-                         double *e_slot, hipStream_t st);
+                         const void *G, const void *GJ, bool g32,
+                         double tai_sign, double *e_slot, hipStream_t st);
 
 // Deterministic compensated sum of the valid slot energies of a batch into
 // partials[4*index + {sum, comp, sum_ct, comp_ct}].
@@ -140,6 +151,9 @@ hipError_t launch_transpose(const double *src, double *dst, int64_t n_outer,
                             int64_t dst_a, int64_t dst_b, double scale,
                             hipStream_t st);
 hipError_t launch_synth(DevSystem &s, uint64_t seed, bool with_j,
+                        hipStream_t st);
+// Fill a Split copy of n doubles.
+hipError_t launch_split(const double *x, DevSystem::Split &out, int64_t n,
                         hipStream_t st);
 
 // Canonical (lexicographic) index of tuple (a,b,c), a<=b<=c, not all equal.

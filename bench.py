@@ -28,7 +28,7 @@ sys.path.insert(0, HERE)
 
 METRIC = ("(T) tuples/s & TFLOP/s, FP64 vs mixed @|ΔE|≤1e-7 Eh; "
           "serve p50/p99")
-PEAK_TFLOPS = {"fp64": 78.6, "split_fp32": 157.3}
+PEAK_TFLOPS = {"fp64": 78.6, "split_fp32": 157.3, "fp32": 157.3}
 
 
 def n_tuples(nv: int) -> int:
@@ -124,7 +124,7 @@ def main(argv=None):
     ap.add_argument("--no", type=int, default=40)
     ap.add_argument("--nv", type=int, default=400)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "split_fp32"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "split_fp32", "fp32"])
     ap.add_argument("--window", type=int, default=8)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--tile", type=int, default=-1)
@@ -132,6 +132,10 @@ def main(argv=None):
                     help="steps that cover the tuple space once")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision comparison pass")
+    ap.add_argument("--mixed", default="split_fp32", choices=["split_fp32", "fp32"],
+                    help="form of the comparison pass")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -171,12 +175,43 @@ def main(argv=None):
     ctx.set_option("timing", 0)
 
     flop_tuple = 12.0 * args.no ** 3 * (args.no + args.nv)
-    kname = "doubles" if args.precision == "fp64" else "doubles_mixed"
+    kname = {"fp64": "doubles", "split_fp32": "doubles_mixed",
+             "fp32": "doubles_fp32"}[args.precision]
     ks = ctx.kernel_stats(kname)
     es = ctx.kernel_stats("energy")
     achieved = ks.flop / (ks.total_ms * 1e-3) / 1e12 if ks.total_ms else 0.0
     peak = PEAK_TFLOPS[args.precision]
     value = done / elapsed
+
+    mixed = None
+    if args.precision == "fp64" and not args.no_mixed:
+        # same steps in split FP32 (mixed form) on the same resident inputs
+        ctx.run(sys_, args.mixed, first=slices[-1][0], count=slices[-1][1])
+        ctx.reset_stats()
+        ctx.set_option("timing", 1)
+        m_sum = 0.0
+        t0m = time.perf_counter()
+        for i in range(args.steps):
+            first, count = slices[i % len(slices)]
+            em, _ = ctx.run(sys_, args.mixed, first=first, count=count)
+            m_sum += em
+        m_elapsed = time.perf_counter() - t0m
+        ctx.set_option("timing", 0)
+        mk = ctx.kernel_stats("doubles_mixed" if args.mixed == "split_fp32"
+                              else "doubles_fp32")
+        m_value = done / m_elapsed
+        mixed = {
+            "form": args.mixed,
+            "value": m_value,
+            "unit": "tuples/s",
+            "speedup_vs_fp64": m_value / value,
+            "tflops": m_value * flop_tuple / 1e12,
+            "doubles_kernel_tflops": mk.flop / (mk.total_ms * 1e-3) / 1e12 if mk.total_ms else 0.0,
+            "doubles_kernel_frac_of_fp32_peak": (mk.flop / (mk.total_ms * 1e-3) / 1e12 / PEAK_TFLOPS["split_fp32"]) if mk.total_ms else 0.0,
+            "energy_Eh": m_sum,
+            "abs_dE_Eh": abs(m_sum - e_sum),
+            "rel_dE": abs(m_sum - e_sum) / abs(e_sum) if e_sum else None,
+        }
 
     cpu = None
     if not args.no_cpu:
@@ -223,6 +258,7 @@ def main(argv=None):
         "tuples_timed": done,
         "energy_Eh": e_sum if full else None,
         "setup_s": gen_s,
+        "mixed_precision": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

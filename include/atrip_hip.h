@@ -43,8 +43,10 @@ typedef struct {
 
 /* Arithmetic of the doubles contractions. */
 enum atrip_precision {
-  ATRIP_FP64 = 0,       /* FP64 MFMA, the plain path */
-  ATRIP_SPLIT_FP32 = 1  /* split-FP32 hi+lo on FP32 MFMA, FP64 energy */
+  ATRIP_FP64 = 0,        /* FP64 MFMA, the plain path */
+  ATRIP_SPLIT_FP32 = 1,  /* split FP32: hi*hi on FP32 MFMA + hi*lo + lo*hi
+                            corrections, FP32 accumulation, FP64 energy */
+  ATRIP_FP32 = 2         /* hi*hi only on FP32 MFMA, FP64 energy */
 };
 
 /* Create a context on HIP device `device`.  Replaces the device selection

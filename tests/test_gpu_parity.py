@@ -161,3 +161,46 @@ def test_errors_are_reported(hip_ctx):
         hip_ctx.set_option("window", 0)
     with pytest.raises(hip.HipError, match="unknown option"):
         hip_ctx.set_option("no-such-knob", 1)
+
+
+# ---- mixed precision (split FP32 doubles, FP64 energy) ----------------------
+
+MIXED_REL_TUPLE = 1e-5   # per tuple, relative to max |E_abc| (FP32 accumulation)
+MIXED_REL_E = 1e-7       # total E(T), relative
+
+
+def test_mixed_precision_against_fp64(hip_ctx, oracle):
+    for no, nv, seed in [(4, 7, 1), (10, 50, 1), (17, 23, 2)]:
+        t = oracle.synthetic(no, nv, seed)
+        e_ref, _, pt_ref, _ = oracle.run(no, nv, t, per_tuple=True)
+        s = hip_ctx.upload(no, nv, t)
+        try:
+            e, ect = hip_ctx.run(s, "split_fp32")
+            n = oracle.n_tuples(nv)
+            pt = hip_ctx.tuple_energies(s, 0, n, precision="split_fp32")
+        finally:
+            hip_ctx.release(s)
+        assert abs(e - e_ref) <= MIXED_REL_E * abs(e_ref)
+        assert ect == e
+        assert np.max(np.abs(pt - pt_ref)) <= MIXED_REL_TUPLE * np.max(np.abs(pt_ref))
+
+
+def test_mixed_precision_cT_and_partition(hip_ctx, oracle):
+    no, nv = 9, 21
+    t = oracle.synthetic(no, nv, 5, with_j=True)
+    e_ref, ect_ref = oracle.run(no, nv, t)
+    s = hip_ctx.upload(no, nv, t)
+    try:
+        e, ect = hip_ctx.run(s, "split_fp32")
+        n = oracle.n_tuples(nv)
+        a = hip_ctx.tuple_energies(s, 0, n, precision="split_fp32")
+        hip_ctx.set_option("window", 4)
+        hip_ctx.set_option("batch", 1)
+        b = hip_ctx.tuple_energies(s, 0, n, precision="split_fp32")
+        hip_ctx.set_option("window", 8)
+        hip_ctx.set_option("batch", 4)
+    finally:
+        hip_ctx.release(s)
+    assert abs(e - e_ref) <= MIXED_REL_E * abs(e_ref)
+    assert abs(ect - ect_ref) <= MIXED_REL_E * abs(ect_ref)
+    assert np.array_equal(a, b)
