@@ -9,36 +9,17 @@
 #include <string>
 #include <vector>
 
-#include "internal.hpp"
+#include "host.hpp"
 
 using namespace atrip_amd;
 
+DevSystem &atrip_amd::get_sys(atrip_hip_ctx *ctx, int sys) {
+  auto it = ctx->systems.find(sys);
+  if (it == ctx->systems.end()) throw Fail{"unknown system handle"};
+  return it->second;
+}
+
 namespace {
-
-struct Fail {
-  std::string msg;
-};
-
-void check(hipError_t e, const char *what) {
-  if (e != hipSuccess)
-    throw Fail{std::string(what) + ": " + hipGetErrorString(e)};
-}
-
-template <typename F>
-int guarded(atrip_hip_ctx *ctx, F &&f) {
-  if (!ctx) return 1;
-  try {
-    ctx->err.clear();
-    f();
-    return 0;
-  } catch (const Fail &x) {
-    ctx->err = x.msg;
-  } catch (const std::exception &x) {
-    ctx->err = x.what();
-  }
-  if (ctx->err.empty()) ctx->err = "unknown failure";
-  return 2;
-}
 
 double *dev_alloc(DevSystem &s, int64_t n, const char *what) {
   void *p = nullptr;
@@ -86,7 +67,22 @@ void free_system(DevSystem &s) {
   s.allocations.clear();
 }
 
-uint64_t n_tuples(uint64_t Nv) { return Nv * (Nv + 1) * (Nv + 2) / 6 - Nv; }
+}  // namespace
+
+// Device bytes alloc_system takes for a system (the arrays below, the
+// packed point table and the canonical-index prefix).
+uint64_t atrip_amd::system_bytes(int64_t No, int64_t Nv, bool with_j) {
+  const uint64_t Kv = pad16(Nv), Ko = pad16(No), N2 = No * No, Nv2 = Nv * Nv;
+  const auto d = [](uint64_t n) { return std::max<uint64_t>(n, 1) * 8; };
+  uint64_t b = d(Nv * N2 * Kv) + d(Nv2 * No * Kv) + d(Nv * N2 * Ko) +
+               2 * d(Nv2 * No * Ko) + d(Nv2 * N2) + d(Nv * No) + d(No) + d(Nv);
+  if (with_j) b += d(Nv2 * No * Kv) + d(Nv * N2 * Ko);
+  b += (uint64_t)No * (No + 1) * (No + 2) / 6 * sizeof(uint32_t);
+  b += (Nv + 1) * sizeof(uint64_t);
+  return b;
+}
+
+namespace {
 
 void alloc_system(atrip_hip_ctx *ctx, DevSystem &s, int64_t No, int64_t Nv,
                   bool with_j) {
@@ -97,18 +93,34 @@ void alloc_system(atrip_hip_ctx *ctx, DevSystem &s, int64_t No, int64_t Nv,
   s.Kv = pad16(Nv);
   s.Ko = pad16(No);
   const int64_t N2 = No * No, Nv2 = Nv * Nv;
-  s.T3 = dev_alloc(s, Nv * N2 * s.Kv, "alloc T3");
-  s.V3 = dev_alloc(s, Nv2 * No * s.Kv, "alloc V3");
-  s.H3 = dev_alloc(s, Nv * N2 * s.Ko, "alloc H3");
-  s.T2N = dev_alloc(s, Nv2 * No * s.Ko, "alloc T2N");
-  s.T2TN = dev_alloc(s, Nv2 * No * s.Ko, "alloc T2TN");
+  s.store = ctx->operand_store;
+  // doubles operands: FP64, or only their split / FP32 form
+  auto operand = [&](double *&x, DevSystem::Split &sp, int64_t n,
+                     const char *what) {
+    if (s.store == 0) {
+      x = dev_alloc(s, n, what);
+      return;
+    }
+    sp.h = (float *)dev_bytes(s, (size_t)n * sizeof(float), what);
+    if (s.store == 1) {
+      sp.b = (uint16_t *)dev_bytes(s, (size_t)n * 2, what);
+      sp.l = (uint16_t *)dev_bytes(s, (size_t)n * 2, what);
+    }
+  };
+  operand(s.T3, s.sT3, Nv * N2 * s.Kv, "alloc T3");
+  operand(s.V3, s.sV3, Nv2 * No * s.Kv, "alloc V3");
+  operand(s.H3, s.sH3, Nv * N2 * s.Ko, "alloc H3");
+  operand(s.T2N, s.sT2N, Nv2 * No * s.Ko, "alloc T2N");
+  operand(s.T2TN, s.sT2TN, Nv2 * No * s.Ko, "alloc T2TN");
+  s.split_ready = s.store != 0;
   s.VHH = dev_alloc(s, Nv2 * N2, "alloc VHH");
   s.Tai = dev_alloc(s, Nv * No, "alloc Tai");
   s.ei = dev_alloc(s, No, "alloc ei");
   s.ea = dev_alloc(s, Nv, "alloc ea");
+  s.has_j = with_j;
   if (with_j) {
-    s.V3J = dev_alloc(s, Nv2 * No * s.Kv, "alloc V3J");
-    s.H3J = dev_alloc(s, Nv * N2 * s.Ko, "alloc H3J");
+    operand(s.V3J, s.sV3J, Nv2 * No * s.Kv, "alloc V3J");
+    operand(s.H3J, s.sH3J, Nv * N2 * s.Ko, "alloc H3J");
   }
   // i >= j >= k points of the energy loops, packed 10 bits per index
   std::vector<uint32_t> pts;
@@ -121,6 +133,7 @@ void alloc_system(atrip_hip_ctx *ctx, DevSystem &s, int64_t No, int64_t Nv,
     void *p = nullptr;
     check(hipMalloc(&p, pts.size() * sizeof(uint32_t)), "alloc points");
     s.allocations.push_back(p);
+    s.bytes += pts.size() * sizeof(uint32_t);
     check(hipMemcpy(p, pts.data(), pts.size() * sizeof(uint32_t),
                     hipMemcpyHostToDevice),
           "upload points");
@@ -136,6 +149,7 @@ void alloc_system(atrip_hip_ctx *ctx, DevSystem &s, int64_t No, int64_t Nv,
     void *p = nullptr;
     check(hipMalloc(&p, pre.size() * sizeof(uint64_t)), "alloc prefix");
     s.allocations.push_back(p);
+    s.bytes += pre.size() * sizeof(uint64_t);
     check(hipMemcpy(p, pre.data(), pre.size() * sizeof(uint64_t),
                     hipMemcpyHostToDevice),
           "upload prefix");
@@ -173,10 +187,53 @@ void staged(atrip_hip_ctx *ctx, const double *host, int64_t slab,
   check(hipFree(stage), "free staging");
 }
 
+// A system that keeps only the split / FP32 form of its doubles operands
+// gets each FP64 operand in a temporary device array, converted and freed
+// as soon as it is filled.
+struct TempOperands {
+  DevSystem &s;
+  hipStream_t st;
+  std::vector<std::pair<double **, DevSystem::Split *>> items;
+  std::vector<int64_t> sizes;
+  void add(double *&x, DevSystem::Split &sp, int64_t n) {
+    if (s.store == 0) return;
+    void *p = nullptr;
+    check(hipMalloc(&p, (size_t)std::max<int64_t>(n, 1) * sizeof(double)),
+          "alloc temporary operand");
+    check(hipMemsetAsync(p, 0, (size_t)std::max<int64_t>(n, 1) * sizeof(double),
+                         st),
+          "clear temporary operand");
+    x = (double *)p;
+    items.push_back({&x, &sp});
+    sizes.push_back(n);
+  }
+  void convert() {
+    for (size_t q = 0; q < items.size(); q++) {
+      check(launch_split(*items[q].first, *items[q].second, sizes[q], st),
+            "split conversion");
+      check(hipStreamSynchronize(st), "split conversion");
+      (void)hipFree(*items[q].first);
+      *items[q].first = nullptr;
+    }
+    items.clear();
+    sizes.clear();
+  }
+  ~TempOperands() {
+    for (auto &it : items) {
+      (void)hipFree(*it.first);
+      *it.first = nullptr;
+    }
+  }
+};
+
 void upload_host(atrip_hip_ctx *ctx, DevSystem &s, const atrip_sys_desc *d) {
   const int64_t No = s.No, Nv = s.Nv, N2 = No * No, Nv2 = Nv * Nv;
   const int64_t Kv = s.Kv, Ko = s.Ko;
   hipStream_t st = ctx->stream;
+  TempOperands tmp{s, st, {}, {}};
+  tmp.add(s.T3, s.sT3, Nv * N2 * Kv);
+  tmp.add(s.T2N, s.sT2N, Nv2 * No * Ko);
+  tmp.add(s.T2TN, s.sT2TN, Nv2 * No * Ko);
   // Tabij {Nv,Nv,No,No}: slowest index j
   staged(ctx, d->Tabij, Nv2 * No, No,
          [&](const double *c, int64_t j0, int64_t cj) {
@@ -218,11 +275,20 @@ void upload_host(atrip_hip_ctx *ctx, DevSystem &s, const atrip_sys_desc *d) {
                    "relayout V3");
            });
   };
+  tmp.convert();
+  tmp.add(s.H3, s.sH3, Nv * N2 * Ko);
   hhh(d->Vijka, s.H3);
+  tmp.convert();
+  tmp.add(s.V3, s.sV3, Nv2 * No * Kv);
   ppph(d->Vabci, s.V3);
-  if (s.V3J) {
+  tmp.convert();
+  if (d->Jijka) {
+    tmp.add(s.H3J, s.sH3J, Nv * N2 * Ko);
     hhh(d->Jijka, s.H3J);
+    tmp.convert();
+    tmp.add(s.V3J, s.sV3J, Nv2 * No * Kv);
     ppph(d->Jabci, s.V3J);
+    tmp.convert();
   }
   check(hipMemcpy(s.Tai, d->Tai, sizeof(double) * Nv * No,
                   hipMemcpyHostToDevice),
@@ -231,12 +297,6 @@ void upload_host(atrip_hip_ctx *ctx, DevSystem &s, const atrip_sys_desc *d) {
         "upload ei");
   check(hipMemcpy(s.ea, d->ea, sizeof(double) * Nv, hipMemcpyHostToDevice),
         "upload ea");
-}
-
-DevSystem &get_sys(atrip_hip_ctx *ctx, int sys) {
-  auto it = ctx->systems.find(sys);
-  if (it == ctx->systems.end()) throw Fail{"unknown system handle"};
-  return it->second;
 }
 
 // Host-side canonical index, identical to the device one.
@@ -317,6 +377,10 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
       precision != ATRIP_FP32)
     throw Fail{"unknown precision " + std::to_string(precision)};
   const bool g32 = precision != ATRIP_FP64;
+  if (precision == ATRIP_FP64 && s.store != 0)
+    throw Fail{"system was uploaded without FP64 operands (operand_store)"};
+  if (precision == ATRIP_SPLIT_FP32 && s.store == 2)
+    throw Fail{"system was uploaded with FP32 operands only (operand_store 2)"};
   if (g32) ensure_split(ctx, s);
   if (first >= last) return res;
   const int nb = plan.nb;
@@ -365,7 +429,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   const int64_t nb3 = (int64_t)nb * nb * nb;
   const int64_t slots_max = (int64_t)std::min(bw, nsel) * nb3;
   const int64_t W3 = w_pad(s.No) * w_pad(s.No) * w_pad(s.No);
-  const bool use_j = s.V3J != nullptr;
+  const bool use_j = s.has_j;
   const size_t gsz =
       (size_t)(3 * slots_max * W3) * (g32 ? sizeof(float) : sizeof(double));
   ensure((void *&)ctx->G, ctx->G_bytes, use_j ? 2 * gsz : gsz, "alloc G");
@@ -531,6 +595,7 @@ int atrip_hip_destroy(atrip_hip_ctx *ctx) {
   if (ctx->G) (void)hipFree(ctx->G);
   if (ctx->e_slot) (void)hipFree(ctx->e_slot);
   if (ctx->partials) (void)hipFree(ctx->partials);
+  if (ctx->bscratch) (void)hipFree(ctx->bscratch);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;
@@ -660,6 +725,28 @@ int atrip_hip_set_option(atrip_hip_ctx *ctx, const char *key, int64_t value) {
     } else if (k == "tile") {
       if (value < -1 || value > 7) throw Fail{"tile must be in -1..7"};
       ctx->tile = (int)value;
+    } else if (k == "operand_store") {
+      if (value < 0 || value > 2) throw Fail{"operand_store must be 0, 1 or 2"};
+      ctx->operand_store = (int)value;
+    } else if (k == "batch_scratch_mb") {
+      if (value < 16 || value > (1 << 18))
+        throw Fail{"batch_scratch_mb must be in 16..262144"};
+      ctx->batch_scratch_mb = value;
+    } else if (k == "batch_grid") {
+      if (value < 0 || value > (1 << 20))
+        throw Fail{"batch_grid must be in 0..1048576"};
+      ctx->batch_grid = (int)value;
+    } else if (k == "batch_order") {
+      if (value < 0 || value > 1) throw Fail{"batch_order must be 0 or 1"};
+      ctx->batch_order = (int)value;
+    } else if (k == "batch_nb") {
+      if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
+        throw Fail{"batch_nb must be 0, 1, 2, 4 or 8"};
+      ctx->batch_nb = (int)value;
+    } else if (k == "batch_tile") {
+      if (value < -1 || value >= N_BTILES)
+        throw Fail{"batch_tile must be in -1..7"};
+      ctx->batch_tile = (int)value;
     } else {
       throw Fail{"unknown option '" + k + "'"};
     }
@@ -674,6 +761,12 @@ int atrip_hip_get_option(atrip_hip_ctx *ctx, const char *key, int64_t *value) {
     else if (k == "batch") *value = ctx->batch;
     else if (k == "timing") *value = ctx->timing;
     else if (k == "tile") *value = ctx->tile;
+    else if (k == "operand_store") *value = ctx->operand_store;
+    else if (k == "batch_scratch_mb") *value = ctx->batch_scratch_mb;
+    else if (k == "batch_grid") *value = ctx->batch_grid;
+    else if (k == "batch_order") *value = ctx->batch_order;
+    else if (k == "batch_nb") *value = ctx->batch_nb;
+    else if (k == "batch_tile") *value = ctx->batch_tile;
     else throw Fail{"unknown option '" + k + "'"};
   });
 }

@@ -213,22 +213,12 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
 
 // x -> hi = (float)x, b = bf16(hi), l = bf16(x - hi); bf16 by round to
 // nearest even on the float bits.
-__device__ inline uint16_t to_bf16(float v) {
-  uint32_t u = __float_as_uint(v);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-
 __global__ __launch_bounds__(256) void split_kernel(const double *x, float *h,
                                                     uint16_t *b, uint16_t *l,
                                                     int64_t n) {
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
-    const double v = x[q];
-    const float hi = (float)v;
-    h[q] = hi;
-    b[q] = to_bf16(hi);
-    l[q] = to_bf16((float)(v - (double)hi));
+    split_store(x[q], h, b, l, q);
   }
 }
 

@@ -1,0 +1,51 @@
+// Host-side helpers shared by the C-ABI translation units: the error
+// convention (every entry point returns 0 or a non-zero status and keeps the
+// message for atrip_hip_last_error) and the resident-system lookup.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <exception>
+#include <string>
+
+#include "internal.hpp"
+
+namespace atrip_amd {
+
+struct Fail {
+  std::string msg;
+};
+
+inline void check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw Fail{std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+template <typename F>
+int guarded(atrip_hip_ctx *ctx, F &&f) {
+  if (!ctx) return 1;
+  try {
+    ctx->err.clear();
+    f();
+    return 0;
+  } catch (const Fail &x) {
+    ctx->err = x.msg;
+  } catch (const std::exception &x) {
+    ctx->err = x.what();
+  }
+  if (ctx->err.empty()) ctx->err = "unknown failure";
+  return 2;
+}
+
+// Resident system of a handle; throws Fail for an unknown handle.
+DevSystem &get_sys(atrip_hip_ctx *ctx, int sys);
+
+// Device bytes of the resident arrays of a system (what admission charges).
+uint64_t system_bytes(int64_t No, int64_t Nv, bool with_j);
+
+// Number of tuples a <= b <= c < Nv, not all equal.
+inline uint64_t n_tuples(uint64_t Nv) {
+  return Nv * (Nv + 1) * (Nv + 2) / 6 - Nv;
+}
+
+}  // namespace atrip_amd

@@ -49,6 +49,7 @@ struct DevSystem {
          *T2TN = nullptr, *VHH = nullptr, *Tai = nullptr, *ei = nullptr,
          *ea = nullptr;
   double *V3J = nullptr, *H3J = nullptr;  // (cT) integrals, may be null
+  bool has_j = false;                     // (cT) integrals present
   // Split-FP32 copies of the GEMM operands, made on the first mixed run:
   // h = (float)x, b = bf16(h), l = bf16(x - h), same layouts as above.
   struct Split {
@@ -57,12 +58,36 @@ struct DevSystem {
   };
   Split sT3, sV3, sH3, sT2N, sT2TN, sV3J, sH3J;
   bool split_ready = false;
+  // Operand store: 0 FP64 (split copies made on the first mixed run),
+  // 1 split FP32 only, 2 FP32 (h) only.  Modes 1 and 2 keep no FP64 copy of
+  // the doubles operands (T3, V3, H3, T2N, T2TN and the J arrays), which
+  // halves (mode 2) the resident bytes of the large systems.
+  int store = 0;
   uint32_t *points = nullptr;             // packed (i>=j>=k) triples
   int64_t npoints = 0;
   uint64_t *aprefix = nullptr;            // tuples with first index < a
   uint64_t bytes = 0;
   std::vector<void *> allocations;
 };
+
+// bf16 of a float, round to nearest even (finite inputs).
+__device__ inline uint16_t to_bf16(float v) {
+  uint32_t u = __float_as_uint(v);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// Split-FP32 element q of x: h = (float)x, b = bf16(h), l = bf16(x - h);
+// b and l may be null (FP32-only store).
+__device__ inline void split_store(double x, float *h, uint16_t *b,
+                                   uint16_t *l, int64_t q) {
+  const float hi = (float)x;
+  h[q] = hi;
+  if (b) {
+    b[q] = to_bf16(hi);
+    l[q] = to_bf16((float)(x - (double)hi));
+  }
+}
 
 // Tuple windows: block triples (alpha <= beta <= gamma) of edge nb.
 struct WindowPlan {
@@ -81,6 +106,13 @@ struct KernelStat {
 
 struct Batch;
 
+// Tuned plan of the batched form for one system shape.
+struct BatchTune {
+  int nb = 0, tile = 0;
+  double sec_window = 0.0;  // measured (or modelled) time of a full window
+  bool measured = false;
+};
+
 }  // namespace atrip_amd
 
 struct atrip_hip_ctx {
@@ -95,6 +127,7 @@ struct atrip_hip_ctx {
   int batch = 4;
   int timing = 0;
   int tile = -1;  // -1: chosen per system shape
+  int operand_store = 0;  // store mode of the next uploads (DevSystem::store)
   // scratch, grown on demand
   double *G = nullptr;       // class buffers [ncls][slots][No^3]
   size_t G_bytes = 0;
@@ -102,6 +135,15 @@ struct atrip_hip_ctx {
   size_t e_slot_bytes = 0;
   double *partials = nullptr;  // per-batch (sum, comp) x 2
   size_t partials_bytes = 0;
+  // batched form: tuned plans keyed by (No, padded Nv), workgroup scratch
+  std::map<std::pair<int64_t, int64_t>, atrip_amd::BatchTune> btune;
+  double *bscratch = nullptr;
+  size_t bscratch_bytes = 0;
+  int64_t batch_scratch_mb = 8192;
+  int batch_grid = 0;   // 0: occupancy x CUs
+  int batch_order = 0;  // 0: longest window first, 1: canonical
+  int batch_nb = 0;     // forced window edge of the batched form (0: tuned)
+  int batch_tile = -1;  // forced tile variant of the batched form (-1: tuned)
   std::map<std::string, atrip_amd::KernelStat> stats;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_events;
   std::vector<std::string> pending_names;
@@ -156,6 +198,52 @@ hipError_t launch_synth(DevSystem &s, uint64_t seed, bool with_j,
 hipError_t launch_split(const double *x, DevSystem::Split &out, int64_t n,
                         hipStream_t st);
 
+// ---- batched form: many small resident systems in one persistent launch --
+
+// One system of a batched launch, with the window edge and tile variant
+// chosen for it.  Tile variant v is the wave tile of (16*bfm(v)) x
+// (16*bfn(v)) outputs.
+struct BSys {
+  const double *T3, *V3, *H3, *T2N, *T2TN, *VHH, *Tai, *ei, *ea;
+  const uint32_t *points;
+  int64_t No, Nv, Kv, Ko, npoints;
+  int32_t nb, tile;
+};
+constexpr int N_BTILES = 8;
+constexpr __host__ __device__ inline int bfm(int v) {
+  constexpr int t[N_BTILES] = {1, 1, 2, 2, 1, 4, 2, 4};
+  return t[v];
+}
+constexpr __host__ __device__ inline int bfn(int v) {
+  constexpr int t[N_BTILES] = {1, 2, 1, 2, 4, 1, 4, 2};
+  return t[v];
+}
+
+// One work item: a window (block triple of edge nb) of one system.
+struct BTask {
+  int32_t sys, al, be, ga;
+  int64_t tuple_off;  // first entry of the task's slots in the per-tuple output
+};
+
+struct BatchedLaunch {
+  const BSys *sys;          // device
+  const BTask *tasks;       // device, canonical order (system-major)
+  const int32_t *order;     // device, dispatch order (a permutation)
+  int ntasks;
+  unsigned *counter;        // device, zero before the launch
+  double *scratch;          // grid * scratch_stride doubles
+  int64_t scratch_stride;
+  double *task_e;           // 2 per task: (sum, compensation)
+  double *tuple_e;          // optional per-slot energies
+  int grid;
+};
+
+// Persistent launch: every workgroup takes windows from the dispatch order
+// until none is left; doubles, singles and energy of a window stay in the
+// workgroup.
+hipError_t launch_batched(const BatchedLaunch &L, hipStream_t st);
+// Workgroups per CU the persistent kernel can hold (for the grid size).
+int batched_blocks_per_cu();
 // Canonical (lexicographic) index of tuple (a,b,c), a<=b<=c, not all equal.
 __host__ __device__ inline uint64_t tuple_index(const uint64_t *aprefix,
                                                 uint64_t Nv, uint64_t a,

@@ -44,7 +44,9 @@ enum SynthTarget { S_T3, S_V3, S_H3, S_T2N, S_T2TN, S_VHH, S_PLAIN };
 
 // Fill one device array from the generator; element q of the destination is
 // mapped back to the CTF linear index of the tensor it holds.
-__global__ __launch_bounds__(256) void synth_kernel(double *dst, int64_t n,
+__global__ __launch_bounds__(256) void synth_kernel(double *dst, float *sh,
+                                                    uint16_t *sb, uint16_t *sl,
+                                                    int64_t n,
                                                     int target, int tensor,
                                                     uint64_t stream, int64_t No,
                                                     int64_t Nv, int64_t Kv,
@@ -93,19 +95,23 @@ __global__ __launch_bounds__(256) void synth_kernel(double *dst, int64_t n,
     if (idx >= 0)
       v = sign * atrip_synth_value(stream, tensor, (uint64_t)No, (uint64_t)Nv,
                                    (uint64_t)idx);
-    dst[q] = v;
+    if (dst) dst[q] = v;
+    if (sh) split_store(v, sh, sb, sl, q);
   }
 }
 
-hipError_t synth_fill(double *dst, int64_t n, SynthTarget target, int tensor,
-                      uint64_t seed, const DevSystem &s, hipStream_t st) {
+// Fill n elements of a target layout into dst (FP64) and/or the split
+// arrays o (o.h null: none).
+hipError_t synth_fill(double *dst, const DevSystem::Split &o, int64_t n,
+                      SynthTarget target, int tensor, uint64_t seed,
+                      const DevSystem &s, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const uint64_t stream = atrip_synth_stream(seed, tensor);
   int64_t blocks = (n + 255) / 256;
   if (blocks > 65536L * 16) blocks = 65536L * 16;
   hipLaunchKernelGGL(synth_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
-                     dst, n, (int)target, tensor, stream, s.No, s.Nv, s.Kv,
-                     s.Ko);
+                     dst, o.h, o.b, o.l, n, (int)target, tensor, stream, s.No,
+                     s.Nv, s.Kv, s.Ko);
   return hipGetLastError();
 }
 
@@ -136,23 +142,31 @@ hipError_t launch_synth(DevSystem &s, uint64_t seed, bool with_j,
                         hipStream_t st) {
   const int64_t No = s.No, Nv = s.Nv, N2 = No * No, Nv2 = Nv * Nv;
   hipError_t e;
-#define SYN(ptr, n, tgt, tensor)                                      \
-  if ((e = synth_fill(ptr, n, tgt, tensor, seed, s, st)) != hipSuccess) \
+  const DevSystem::Split none;
+  // doubles operands: FP64 array, or straight into the split arrays when
+  // the system keeps no FP64 copy
+#define SYN(ptr, sp, n, tgt, tensor)                                        \
+  if ((e = synth_fill(s.store ? nullptr : ptr, s.store ? sp : none, n, tgt, \
+                      tensor, seed, s, st)) != hipSuccess)                  \
     return e;
-  SYN(s.T3, Nv * N2 * s.Kv, S_T3, ATRIP_SYN_TABIJ);
-  SYN(s.V3, Nv2 * No * s.Kv, S_V3, ATRIP_SYN_VABCI);
-  SYN(s.H3, Nv * N2 * s.Ko, S_H3, ATRIP_SYN_VIJKA);
-  SYN(s.T2N, Nv2 * No * s.Ko, S_T2N, ATRIP_SYN_TABIJ);
-  SYN(s.T2TN, Nv2 * No * s.Ko, S_T2TN, ATRIP_SYN_TABIJ);
-  SYN(s.VHH, Nv2 * N2, S_VHH, ATRIP_SYN_VABIJ);
-  SYN(s.Tai, Nv * No, S_PLAIN, ATRIP_SYN_TAI);
-  SYN(s.ei, No, S_PLAIN, ATRIP_SYN_EI);
-  SYN(s.ea, Nv, S_PLAIN, ATRIP_SYN_EA);
+#define SYN64(ptr, n, tgt, tensor)                                         \
+  if ((e = synth_fill(ptr, none, n, tgt, tensor, seed, s, st)) != hipSuccess) \
+    return e;
+  SYN(s.T3, s.sT3, Nv * N2 * s.Kv, S_T3, ATRIP_SYN_TABIJ);
+  SYN(s.V3, s.sV3, Nv2 * No * s.Kv, S_V3, ATRIP_SYN_VABCI);
+  SYN(s.H3, s.sH3, Nv * N2 * s.Ko, S_H3, ATRIP_SYN_VIJKA);
+  SYN(s.T2N, s.sT2N, Nv2 * No * s.Ko, S_T2N, ATRIP_SYN_TABIJ);
+  SYN(s.T2TN, s.sT2TN, Nv2 * No * s.Ko, S_T2TN, ATRIP_SYN_TABIJ);
+  SYN64(s.VHH, Nv2 * N2, S_VHH, ATRIP_SYN_VABIJ);
+  SYN64(s.Tai, Nv * No, S_PLAIN, ATRIP_SYN_TAI);
+  SYN64(s.ei, No, S_PLAIN, ATRIP_SYN_EI);
+  SYN64(s.ea, Nv, S_PLAIN, ATRIP_SYN_EA);
   if (with_j) {
-    SYN(s.V3J, Nv2 * No * s.Kv, S_V3, ATRIP_SYN_JABCI);
-    SYN(s.H3J, Nv * N2 * s.Ko, S_H3, ATRIP_SYN_JIJKA);
+    SYN(s.V3J, s.sV3J, Nv2 * No * s.Kv, S_V3, ATRIP_SYN_JABCI);
+    SYN(s.H3J, s.sH3J, Nv * N2 * s.Ko, S_H3, ATRIP_SYN_JIJKA);
   }
 #undef SYN
+#undef SYN64
   return hipSuccess;
 }
 

@@ -27,6 +27,11 @@ SYMBOLS = (
     "atrip_hip_run", "atrip_hip_tuple_energies", "atrip_hip_set_option",
     "atrip_hip_get_option", "atrip_hip_kernel_stats", "atrip_hip_reset_stats",
     "atrip_hip_sys_info", "atrip_hip_device_count",
+    "atrip_hip_batch_run", "atrip_hip_batch_tune", "atrip_hip_batch_plan",
+    "atrip_hip_batch_model", "atrip_hip_serve_create", "atrip_hip_serve_destroy",
+    "atrip_hip_serve_set_policy", "atrip_hip_serve_clock_ms",
+    "atrip_hip_serve_submit_synthetic", "atrip_hip_serve_submit",
+    "atrip_hip_serve_step", "atrip_hip_serve_result", "atrip_hip_serve_stats",
 )
 
 TENSORS = ("ei", "ea", "Tai", "Tabij", "Vabij", "Vijka", "Vabci", "Jijka", "Jabci")
@@ -81,12 +86,44 @@ def load():
         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.atrip_hip_device_count.argtypes = []
     L.atrip_hip_device_count.restype = C.c_int
+    ip = C.POINTER(C.c_int)
+    L.atrip_hip_batch_run.argtypes = [vp, ip, C.c_int, C.POINTER(C.c_double)]
+    L.atrip_hip_batch_tune.argtypes = [vp, ip, C.c_int, C.c_int]
+    L.atrip_hip_batch_plan.argtypes = [vp, C.c_int, ip, ip, ip, ip,
+                                       C.POINTER(C.c_double)]
+    L.atrip_hip_batch_model.argtypes = [C.c_int64, C.c_int64, C.c_int64,
+                                        C.c_int, ip, ip, ip]
+    dp = C.POINTER(C.c_double)
+    u64p = C.POINTER(C.c_uint64)
+    L.atrip_hip_serve_create.argtypes = [vp, C.c_uint64, C.c_int, C.c_double,
+                                         C.POINTER(vp)]
+    L.atrip_hip_serve_destroy.argtypes = [vp]
+    L.atrip_hip_serve_set_policy.argtypes = [vp, C.c_int]
+    L.atrip_hip_serve_clock_ms.argtypes = [vp]
+    L.atrip_hip_serve_clock_ms.restype = C.c_double
+    L.atrip_hip_serve_submit_synthetic.argtypes = [
+        vp, C.c_int64, C.c_int64, C.c_uint64, C.c_double, C.POINTER(C.c_int64)]
+    L.atrip_hip_serve_submit.argtypes = [vp, C.POINTER(SysDesc), C.c_double,
+                                         C.POINTER(C.c_int64)]
+    L.atrip_hip_serve_step.argtypes = [vp, ip]
+    L.atrip_hip_serve_result.argtypes = [vp, C.c_int64, ip, dp, dp, dp, dp]
+    L.atrip_hip_serve_stats.argtypes = [vp, ip, ip, u64p, u64p, u64p, u64p, dp]
     _lib = L
     return L
 
 
 def device_count() -> int:
     return int(load().atrip_hip_device_count())
+
+
+def batch_model(no: int, nv: int, scratch_mb: int = 8192, grid: int = 256):
+    """Cost-model plan (window edge, fm, fn) of the batched form; host only."""
+    nb, fm, fn = C.c_int(), C.c_int(), C.c_int()
+    rc = load().atrip_hip_batch_model(no, nv, scratch_mb, grid, C.byref(nb),
+                                      C.byref(fm), C.byref(fn))
+    if rc != 0:
+        raise HipError(f"atrip_hip_batch_model({no}, {nv}) failed ({rc})")
+    return nb.value, fm.value, fn.value
 
 
 def _p(a):
@@ -190,6 +227,29 @@ class Context:
             _p(e), _p(ect) if with_ct else C.POINTER(C.c_double)()),
             "tuple_energies")
         return (e[:count], ect[:count]) if with_ct else e[:count]
+
+    # -- batched form ----------------------------------------------------
+    def batch_run(self, systems) -> np.ndarray:
+        """E(T) of every listed resident system, all in one persistent launch."""
+        arr = (C.c_int * max(len(systems), 1))(*systems)
+        e = np.zeros(max(len(systems), 1))
+        self._check(self._L.atrip_hip_batch_run(self._h, arr, len(systems), _p(e)),
+                    "batch_run")
+        return e[:len(systems)]
+
+    def batch_tune(self, systems, measure=True):
+        arr = (C.c_int * max(len(systems), 1))(*systems)
+        self._check(self._L.atrip_hip_batch_tune(self._h, arr, len(systems),
+                                                 1 if measure else 0), "batch_tune")
+
+    def batch_plan(self, sys_: int) -> dict:
+        nb, fm, fn, meas = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        sec = C.c_double()
+        self._check(self._L.atrip_hip_batch_plan(
+            self._h, sys_, C.byref(nb), C.byref(fm), C.byref(fn), C.byref(meas),
+            C.byref(sec)), "batch_plan")
+        return dict(nb=nb.value, fm=fm.value, fn=fn.value,
+                    measured=bool(meas.value), sec_window=sec.value)
 
     # -- knobs -----------------------------------------------------------
     def set_option(self, key: str, value: int):

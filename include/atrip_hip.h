@@ -103,7 +103,17 @@ int atrip_hip_tuple_energies(atrip_hip_ctx *ctx, int sys, int precision,
  *   "window"  block edge nb of the tuple windows (1..32)
  *   "batch"   windows per launch
  *   "timing"  1: time every hot-kernel launch with HIP events
- *   "tile"    doubles-kernel tile variant 0..7, -1 (default) = automatic */
+ *   "tile"    doubles-kernel tile variant 0..7, -1 (default) = automatic
+ *   "operand_store"  how the next uploads keep the doubles operands:
+ *                    0 FP64 (split copies made by the first mixed run),
+ *                    1 split FP32 only, 2 FP32 only (no FP64 copy: for
+ *                    systems whose FP64 and split copies do not fit together)
+ *   "batch_scratch_mb"  device scratch bound of the batched form (MiB)
+ *   "batch_grid"        batched persistent grid (0 = occupancy x CUs)
+ *   "batch_order"       batched dispatch: 0 longest window first,
+ *                       1 canonical order
+ *   "batch_nb", "batch_tile"  force the batched window edge (0 = tuned) and
+ *                       tile variant (-1 = tuned) */
 int atrip_hip_set_option(atrip_hip_ctx *ctx, const char *key, int64_t value);
 int atrip_hip_get_option(atrip_hip_ctx *ctx, const char *key, int64_t *value);
 
@@ -122,6 +132,84 @@ int atrip_hip_sys_info(atrip_hip_ctx *ctx, int sys, int64_t *No, int64_t *Nv,
 
 /* Number of HIP devices visible (0 when none); never fails. */
 int atrip_hip_device_count(void);
+
+/*
+ * Batched form: the full tuple spaces of nsys resident systems (plain FP64,
+ * No <= 64) in one persistent launch; energies[s] receives E(T) of
+ * systems[s] with the sign of src/atrip/Atrip.cxx:1108-1111.  No
+ * counterpart in the reference, which runs one system per atrip::Atrip::run
+ * (src/atrip/Atrip.cxx:686-1111); each system's result is that of
+ * atrip_hip_run on it (same formulas; summation order of the windows
+ * differs), and does not depend on the other systems of the batch.
+ * Window edge and tile variant of each system come from the tuner (below);
+ * shapes not tuned yet use the cost model.
+ */
+int atrip_hip_batch_run(atrip_hip_ctx *ctx, const int *systems, int nsys,
+                        double *energies);
+
+/*
+ * Autotuner of the batched form.  mode 0: cost model only; mode 1: for each
+ * distinct shape (No, Nv rounded up to 16) among the systems, time the six
+ * (window edge x tile variant) candidates the cost model ranks best, with
+ * the persistent kernel itself on one window per workgroup of that shape,
+ * and keep the fastest per tuple.  The choices are kept in the context and
+ * used by atrip_hip_batch_run and the serving loop.
+ */
+int atrip_hip_batch_tune(atrip_hip_ctx *ctx, const int *systems, int nsys,
+                         int mode);
+
+/* Plan used for a resident system: window edge, tile variant (wave tile of
+ * 16*fm x 16*fn outputs), whether it was measured, and the estimated
+ * seconds per full window. */
+int atrip_hip_batch_plan(atrip_hip_ctx *ctx, int sys, int *nb, int *fm,
+                         int *fn, int *measured, double *sec_window);
+
+/* Cost-model choice for a shape, on the host only (no device needed). */
+int atrip_hip_batch_model(int64_t No, int64_t Nv, int64_t scratch_mb,
+                          int grid, int *nb, int *fm, int *fn);
+
+/*
+ * Serving form: a request loop with continuous batching over one context.
+ * No counterpart in the reference (one Atrip::run per process).  Requests
+ * wait in a FIFO queue and are admitted while a slot is free and their
+ * resident arrays fit hbm_budget_bytes; each atrip_hip_serve_step admits
+ * what fits and runs one persistent launch over windows of every active
+ * request (shortest remaining work first by default), about iter_ms long,
+ * then retires finished requests and frees their memory.  A request's
+ * energy does not depend on the mix it was served in.  The server uses its
+ * context's stream and must be driven from the context's thread; the
+ * context must outlive it.  Errors are reported on the context.
+ */
+typedef struct atrip_hip_server atrip_hip_server;
+
+int atrip_hip_serve_create(atrip_hip_ctx *ctx, uint64_t hbm_budget_bytes,
+                           int max_active, double iter_ms,
+                           atrip_hip_server **out);
+int atrip_hip_serve_destroy(atrip_hip_server *srv);
+/* 0: shortest remaining work first (default), 1: equal share per request */
+int atrip_hip_serve_set_policy(atrip_hip_server *srv, int policy);
+/* Milliseconds since the server was created (the clock of all times). */
+double atrip_hip_serve_clock_ms(atrip_hip_server *srv);
+/* Enqueue a seeded synthetic system (generated on the device when admitted)
+ * or host tensors (read when admitted: they must stay valid until then).
+ * arrival_ms < 0 stamps the request with the current clock. */
+int atrip_hip_serve_submit_synthetic(atrip_hip_server *srv, int64_t No,
+                                     int64_t Nv, uint64_t seed,
+                                     double arrival_ms, int64_t *request);
+int atrip_hip_serve_submit(atrip_hip_server *srv, const atrip_sys_desc *desc,
+                           double arrival_ms, int64_t *request);
+/* Admission, one launch, retirement; *finished = requests ended by it. */
+int atrip_hip_serve_step(atrip_hip_server *srv, int *finished);
+/* State (0 queued, 1 active, 2 done, 3 failed), E(T) when done, and the
+ * arrival, admission and completion times in ms.  Fails for a failed
+ * request, with its reason. */
+int atrip_hip_serve_result(atrip_hip_server *srv, int64_t request, int *state,
+                           double *energy, double *t_submit_ms,
+                           double *t_admit_ms, double *t_done_ms);
+int atrip_hip_serve_stats(atrip_hip_server *srv, int *queued, int *active,
+                          uint64_t *resident_bytes, uint64_t *peak_bytes,
+                          uint64_t *tuples_done, uint64_t *launches,
+                          double *busy_ms);
 
 #ifdef __cplusplus
 }

@@ -204,3 +204,38 @@ def test_mixed_precision_cT_and_partition(hip_ctx, oracle):
     assert abs(e - e_ref) <= MIXED_REL_E * abs(e_ref)
     assert abs(ect - ect_ref) <= MIXED_REL_E * abs(ect_ref)
     assert np.array_equal(a, b)
+
+
+def test_operand_store_modes(hip_ctx, oracle):
+    """Systems kept without FP64 operands (split or FP32 only) give the same
+    bits as the split copies of an FP64-resident system; FP64 runs on them
+    are refused."""
+    from atrip_amd import hip
+
+    no, nv, seed = 7, 19, 11
+    t = oracle.synthetic(no, nv, seed, with_j=True)
+    ref = hip_ctx.upload_synthetic(no, nv, seed, with_j=True)
+    try:
+        want = {p: hip_ctx.run(ref, p) for p in ("split_fp32", "fp32")}
+        for store, precs in ((1, ("split_fp32", "fp32")), (2, ("fp32",))):
+            hip_ctx.set_option("operand_store", store)
+            try:
+                sys_dev = hip_ctx.upload_synthetic(no, nv, seed, with_j=True)
+                sys_host = hip_ctx.upload(no, nv, t)
+            finally:
+                hip_ctx.set_option("operand_store", 0)
+            try:
+                for p in precs:
+                    assert hip_ctx.run(sys_dev, p) == want[p], (store, p)
+                    assert hip_ctx.run(sys_host, p) == want[p], (store, p)
+                with pytest.raises(hip.HipError, match="FP64 operands"):
+                    hip_ctx.run(sys_dev, "fp64")
+                if store == 2:
+                    with pytest.raises(hip.HipError, match="FP32 operands only"):
+                        hip_ctx.run(sys_dev, "split_fp32")
+                assert hip_ctx.sys_info(sys_dev)["bytes"] < hip_ctx.sys_info(ref)["bytes"]
+            finally:
+                hip_ctx.release(sys_dev)
+                hip_ctx.release(sys_host)
+    finally:
+        hip_ctx.release(ref)
