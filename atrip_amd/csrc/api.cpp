@@ -20,28 +20,12 @@ DevSystem &atrip_amd::get_sys(atrip_hip_ctx *ctx, int sys) {
 }
 
 namespace {
-
-double *dev_alloc(DevSystem &s, int64_t n, const char *what) {
-  void *p = nullptr;
-  const size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(double);
-  check(hipMalloc(&p, bytes), what);
-  s.allocations.push_back(p);
-  s.bytes += bytes;
-  check(hipMemset(p, 0, bytes), what);
-  return (double *)p;
-}
-
-void *dev_bytes(DevSystem &s, size_t bytes, const char *what) {
-  void *p = nullptr;
-  check(hipMalloc(&p, std::max<size_t>(bytes, 1)), what);
-  s.allocations.push_back(p);
-  s.bytes += bytes;
-  return p;
+void *dev_bytes(DevSystem &s, size_t bytes, const char *what);
 }
 
 // Split-FP32 copies of the GEMM operands, made once per system on the first
 // mixed-precision run.
-void ensure_split(atrip_hip_ctx *ctx, DevSystem &s) {
+void atrip_amd::ensure_split(atrip_hip_ctx *ctx, DevSystem &s) {
   if (s.split_ready) return;
   const int64_t No = s.No, Nv = s.Nv, N2 = No * No, Nv2 = Nv * Nv;
   auto make = [&](const double *x, DevSystem::Split &o, int64_t n) {
@@ -62,6 +46,26 @@ void ensure_split(atrip_hip_ctx *ctx, DevSystem &s) {
   s.split_ready = true;
 }
 
+namespace {
+
+double *dev_alloc(DevSystem &s, int64_t n, const char *what) {
+  void *p = nullptr;
+  const size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(double);
+  check(hipMalloc(&p, bytes), what);
+  s.allocations.push_back(p);
+  s.bytes += bytes;
+  check(hipMemset(p, 0, bytes), what);
+  return (double *)p;
+}
+
+void *dev_bytes(DevSystem &s, size_t bytes, const char *what) {
+  void *p = nullptr;
+  check(hipMalloc(&p, std::max<size_t>(bytes, 1)), what);
+  s.allocations.push_back(p);
+  s.bytes += bytes;
+  return p;
+}
+
 void free_system(DevSystem &s) {
   for (void *p : s.allocations) (void)hipFree(p);
   s.allocations.clear();
@@ -71,12 +75,18 @@ void free_system(DevSystem &s) {
 
 // Device bytes alloc_system takes for a system (the arrays below, the
 // packed point table and the canonical-index prefix).
-uint64_t atrip_amd::system_bytes(int64_t No, int64_t Nv, bool with_j) {
+uint64_t atrip_amd::system_bytes(int64_t No, int64_t Nv, bool with_j,
+                                 int store) {
   const uint64_t Kv = pad16(Nv), Ko = pad16(No), N2 = No * No, Nv2 = Nv * Nv;
   const auto d = [](uint64_t n) { return std::max<uint64_t>(n, 1) * 8; };
-  uint64_t b = d(Nv * N2 * Kv) + d(Nv2 * No * Kv) + d(Nv * N2 * Ko) +
-               2 * d(Nv2 * No * Ko) + d(Nv2 * N2) + d(Nv * No) + d(No) + d(Nv);
-  if (with_j) b += d(Nv2 * No * Kv) + d(Nv * N2 * Ko);
+  // doubles operands: 8 bytes per element as FP64 or as split (4 + 2 + 2),
+  // 4 as FP32 only
+  const auto o = [&](uint64_t n) {
+    return store == 2 ? n * 4 : store == 1 ? n * 8 : d(n);
+  };
+  uint64_t b = o(Nv * N2 * Kv) + o(Nv2 * No * Kv) + o(Nv * N2 * Ko) +
+               2 * o(Nv2 * No * Ko) + d(Nv2 * N2) + d(Nv * No) + d(No) + d(Nv);
+  if (with_j) b += o(Nv2 * No * Kv) + o(Nv * N2 * Ko);
   b += (uint64_t)No * (No + 1) * (No + 2) / 6 * sizeof(uint32_t);
   b += (Nv + 1) * sizeof(uint64_t);
   return b;

@@ -48,13 +48,23 @@ static double n_windows(int64_t Nv, int nb) {
 // wave tiles of the three classes (per 16-deep K chunk, 4*fm*fn MFMA of 64
 // cycles or the chunk's fm+fn fragment loads, whichever is longer, plus a
 // memory latency per K segment) and the energies (one wave per tuple).
-double window_cycles(int64_t No, int64_t Nv, int nb, int tile) {
+double window_cycles(int64_t No, int64_t Nv, int nb, int tile,
+                     int precision) {
   const double fm = bfm(tile), fn = bfn(tile);
+  // MFMA cycles per 16-deep chunk and 16x16 output fragment: 4 FP64 16x16x4
+  // (64 cycles each); 4 FP32 16x16x4 (32 each), plus one bf16 16x16x32 (16)
+  // for the split corrections
+  const double mf = precision == ATRIP_FP64         ? 4 * 64.0
+                    : precision == ATRIP_SPLIT_FP32 ? 4 * 32.0 + 16.0
+                                                    : 4 * 32.0;
+  // fragment loads per chunk: 32 B/lane FP64, 16 B FP32 (+16 B bf16 pairs)
+  const double ld = precision == ATRIP_FP64 ? 180.0
+                    : precision == ATRIP_SPLIT_FP32 ? 180.0 : 120.0;
   const double N2 = (double)No * No;
   const double mt = std::ceil(N2 / (16 * fm)), nt = std::ceil(nb * No / (16 * fn));
   const double subs = 3.0 * nb * nb * mt * nt;
   const double chunks = (2.0 * pad16(Nv) + 2.0 * pad16(No)) / 16.0;
-  const double per_chunk = std::max(4.0 * fm * fn * 64.0, 180.0 * (fm + fn));
+  const double per_chunk = std::max(mf * fm * fn, ld * (fm + fn));
   const double sub = 4 * 1200.0 + chunks * per_chunk + 300.0 * fm * fn;
   const double npts = (double)No * (No + 1) * (No + 2) / 6.0;
   const double energy = nb * nb * nb / 4.0 * (std::ceil(npts / 64.0) * 400.0 + 2000.0);
@@ -68,26 +78,28 @@ bool nb_fits(int64_t No, int nb, int64_t scratch_mb, int grid) {
 
 // Modelled seconds per tuple of the whole system for (nb, tile): partially
 // filled windows are charged half their empty slots.
-double model_sec_tuple(int64_t No, int64_t Nv, int nb, int tile) {
+double model_sec_tuple(int64_t No, int64_t Nv, int nb, int tile,
+                       int precision) {
   const double nt = (double)n_tuples((uint64_t)Nv);
   const double nwin = n_windows(Nv, nb);
-  const double cyc = window_cycles(No, Nv, nb, tile);
+  const double cyc = window_cycles(No, Nv, nb, tile, precision);
   const double total = cyc * (0.5 * nwin + 0.5 * nt / ((double)nb * nb * nb));
   return total / nt / CLOCK_HZ;
 }
 
-BatchTune model_pick(int64_t No, int64_t Nv, int64_t scratch_mb, int grid) {
+BatchTune model_pick(int64_t No, int64_t Nv, int64_t scratch_mb, int grid,
+                     int precision) {
   BatchTune best;
   double best_s = 1e300;
   for (int nb : NB_CAND) {
     if (!nb_fits(No, nb, scratch_mb, grid)) continue;
     for (int t = 0; t < N_BTILES; t++) {
-      const double s = model_sec_tuple(No, Nv, nb, t);
+      const double s = model_sec_tuple(No, Nv, nb, t, precision);
       if (s < best_s) {
         best_s = s;
         best.nb = nb;
         best.tile = t;
-        best.sec_window = window_cycles(No, Nv, nb, t) / CLOCK_HZ;
+        best.sec_window = window_cycles(No, Nv, nb, t, precision) / CLOCK_HZ;
       }
     }
   }
@@ -95,44 +107,55 @@ BatchTune model_pick(int64_t No, int64_t Nv, int64_t scratch_mb, int grid) {
   return best;
 }
 
-static std::pair<int64_t, int64_t> shape_key(const DevSystem &s) {
-  return {s.No, pad16(s.Nv)};
+static std::array<int64_t, 3> shape_key(const DevSystem &s, int precision) {
+  return {s.No, pad16(s.Nv), precision};
 }
 
-int default_grid(atrip_hip_ctx *ctx) {
+void check_precision(int precision) {
+  if (precision != ATRIP_FP64 && precision != ATRIP_SPLIT_FP32 &&
+      precision != ATRIP_FP32)
+    throw Fail{"unknown precision " + std::to_string(precision)};
+}
+
+int default_grid(atrip_hip_ctx *ctx, int precision) {
   int cus = 0;
   check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                               ctx->device),
         "device attribute");
-  return std::max(1, cus) * batched_blocks_per_cu();
+  return std::max(1, cus) * batched_blocks_per_cu(precision);
 }
 
-BatchTune plan_for(atrip_hip_ctx *ctx, const DevSystem &s, int grid) {
+BatchTune plan_for(atrip_hip_ctx *ctx, const DevSystem &s, int grid,
+                   int precision) {
   BatchTune p;
-  auto it = ctx->btune.find(shape_key(s));
+  auto it = ctx->btune.find(shape_key(s, precision));
   if (it != ctx->btune.end() &&
       nb_fits(s.No, it->second.nb, ctx->batch_scratch_mb, grid))
     p = it->second;
   else
-    p = model_pick(s.No, s.Nv, ctx->batch_scratch_mb, grid);
+    p = model_pick(s.No, s.Nv, ctx->batch_scratch_mb, grid, precision);
   // forced choices (tests, sweeps); the run shrinks the grid if the forced
   // window edge needs more scratch than the bound allows
   if (ctx->batch_nb > 0 && ctx->batch_nb != p.nb) {
     p.nb = ctx->batch_nb;
     p.measured = false;
-    p.sec_window = window_cycles(s.No, s.Nv, p.nb, p.tile) / CLOCK_HZ;
+    p.sec_window = window_cycles(s.No, s.Nv, p.nb, p.tile, precision) / CLOCK_HZ;
   }
   if (ctx->batch_tile >= 0 && ctx->batch_tile != p.tile) {
     p.tile = ctx->batch_tile;
     p.measured = false;
-    p.sec_window = window_cycles(s.No, s.Nv, p.nb, p.tile) / CLOCK_HZ;
+    p.sec_window = window_cycles(s.No, s.Nv, p.nb, p.tile, precision) / CLOCK_HZ;
   }
   return p;
 }
 
-BSys make_bsys(const DevSystem &s, const BatchTune &p) {
-  if (s.store != 0)
-    throw Fail{"the batched form needs FP64 operands (operand_store 0)"};
+BSys make_bsys(atrip_hip_ctx *ctx, DevSystem &s, const BatchTune &p,
+               int precision) {
+  if (precision == ATRIP_FP64 && s.store != 0)
+    throw Fail{"system was uploaded without FP64 operands (operand_store)"};
+  if (precision == ATRIP_SPLIT_FP32 && s.store == 2)
+    throw Fail{"system was uploaded with FP32 operands only (operand_store 2)"};
+  if (precision != ATRIP_FP64) ensure_split(ctx, s);
   BSys b;
   b.T3 = s.T3; b.V3 = s.V3; b.H3 = s.H3; b.T2N = s.T2N; b.T2TN = s.T2TN;
   b.VHH = s.VHH; b.Tai = s.Tai; b.ei = s.ei; b.ea = s.ea;
@@ -141,6 +164,17 @@ BSys make_bsys(const DevSystem &s, const BatchTune &p) {
   b.nb = p.nb;
   b.tile = p.tile;
   return b;
+}
+
+BSplit make_bsplit(const DevSystem &s) {
+  BSplit x;
+  x.T3h = s.sT3.h; x.V3h = s.sV3.h; x.H3h = s.sH3.h;
+  x.T2Nh = s.sT2N.h; x.T2TNh = s.sT2TN.h;
+  x.T3b = s.sT3.b; x.V3b = s.sV3.b; x.H3b = s.sH3.b;
+  x.T2Nb = s.sT2N.b; x.T2TNb = s.sT2TN.b;
+  x.T3l = s.sT3.l; x.V3l = s.sV3.l; x.H3l = s.sH3.l;
+  x.T2Nl = s.sT2N.l; x.T2TNl = s.sT2TN.l;
+  return x;
 }
 
 namespace {
@@ -180,7 +214,9 @@ void neumaier_add(double &sum, double &comp, double s, double c) {
   comp += c;
 }
 
-LaunchOut run_tasks(atrip_hip_ctx *ctx, const std::vector<BSys> &sys,
+LaunchOut run_tasks(atrip_hip_ctx *ctx, int precision,
+                    const std::vector<BSys> &sys,
+                    const std::vector<BSplit> &split,
                     const std::vector<BTask> &tasks,
                     const std::vector<int32_t> &order, int grid, bool timed) {
   LaunchOut out;
@@ -205,9 +241,12 @@ LaunchOut run_tasks(atrip_hip_ctx *ctx, const std::vector<BSys> &sys,
     ctx->bscratch_bytes = need;
   }
   hipStream_t st = ctx->stream;
-  DevBuf dsys, dtasks, dorder, dtask_e, dcounter;
+  DevBuf dsys, dsplit, dtasks, dorder, dtask_e, dcounter;
   BatchedLaunch L;
+  L.precision = precision;
   L.sys = dsys.put(sys, st, "upload batch systems");
+  L.split = precision == ATRIP_FP64 ? nullptr
+                                    : dsplit.put(split, st, "upload batch splits");
   L.tasks = dtasks.put(tasks, st, "upload batch tasks");
   L.order = dorder.put(order, st, "upload batch order");
   L.ntasks = (int)tasks.size();
@@ -245,23 +284,29 @@ LaunchOut run_tasks(atrip_hip_ctx *ctx, const std::vector<BSys> &sys,
 
 extern "C" {
 
-int atrip_hip_batch_model(int64_t No, int64_t Nv, int64_t scratch_mb,
-                          int grid, int *nb, int *fm, int *fn) {
+int atrip_hip_batch_model(int64_t No, int64_t Nv, int precision,
+                          int64_t scratch_mb, int grid, int *nb, int *fm,
+                          int *fn) {
   if (No < 1 || Nv < 1 || grid < 1 || scratch_mb < 1) return 1;
-  const BatchTune p = model_pick(No, Nv, scratch_mb, grid);
+  if (precision != ATRIP_FP64 && precision != ATRIP_SPLIT_FP32 &&
+      precision != ATRIP_FP32)
+    return 1;
+  const BatchTune p = model_pick(No, Nv, scratch_mb, grid, precision);
   if (nb) *nb = p.nb;
   if (fm) *fm = bfm(p.tile);
   if (fn) *fn = bfn(p.tile);
   return 0;
 }
 
-int atrip_hip_batch_plan(atrip_hip_ctx *ctx, int sys, int *nb, int *fm,
-                         int *fn, int *measured, double *sec_window) {
+int atrip_hip_batch_plan(atrip_hip_ctx *ctx, int sys, int precision, int *nb,
+                         int *fm, int *fn, int *measured, double *sec_window) {
   return guarded(ctx, [&] {
+    check_precision(precision);
     const DevSystem &s = get_sys(ctx, sys);
     check(hipSetDevice(ctx->device), "hipSetDevice");
-    const int grid = ctx->batch_grid ? ctx->batch_grid : default_grid(ctx);
-    const BatchTune p = plan_for(ctx, s, grid);
+    const int grid =
+        ctx->batch_grid ? ctx->batch_grid : default_grid(ctx, precision);
+    const BatchTune p = plan_for(ctx, s, grid, precision);
     if (nb) *nb = p.nb;
     if (fm) *fm = bfm(p.tile);
     if (fn) *fn = bfn(p.tile);
@@ -271,23 +316,27 @@ int atrip_hip_batch_plan(atrip_hip_ctx *ctx, int sys, int *nb, int *fm,
 }
 
 int atrip_hip_batch_run(atrip_hip_ctx *ctx, const int *systems, int nsys,
-                        double *energies) {
+                        int precision, double *energies) {
   return guarded(ctx, [&] {
     if (nsys < 0 || (nsys > 0 && (!systems || !energies)))
       throw Fail{"bad system list"};
+    check_precision(precision);
     check(hipSetDevice(ctx->device), "hipSetDevice");
-    const int grid = ctx->batch_grid ? ctx->batch_grid : default_grid(ctx);
+    const int grid =
+        ctx->batch_grid ? ctx->batch_grid : default_grid(ctx, precision);
     std::vector<BSys> sys;
+    std::vector<BSplit> split;
     std::vector<BTask> tasks;
     std::vector<double> cost;
     std::vector<int64_t> begin{0};
     double flop = 0.0;
     for (int q = 0; q < nsys; q++) {
-      const DevSystem &s = get_sys(ctx, systems[q]);
+      DevSystem &s = get_sys(ctx, systems[q]);
       if (s.No > MAX_NO)
         throw Fail{"batched form needs No <= 64 (use atrip_hip_run)"};
-      const BatchTune p = plan_for(ctx, s, grid);
-      sys.push_back(make_bsys(s, p));
+      const BatchTune p = plan_for(ctx, s, grid, precision);
+      sys.push_back(make_bsys(ctx, s, p, precision));
+      split.push_back(make_bsplit(s));
       const int64_t nb3 = (int64_t)p.nb * p.nb * p.nb;
       int64_t off = 0;
       for (const Win &w : windows_of(s.Nv, p.nb)) {
@@ -305,7 +354,8 @@ int atrip_hip_batch_run(atrip_hip_ctx *ctx, const int *systems, int nsys,
     if (ctx->batch_order == 0)
       std::stable_sort(order.begin(), order.end(),
                        [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
-    LaunchOut r = run_tasks(ctx, sys, tasks, order, grid, ctx->timing);
+    LaunchOut r =
+        run_tasks(ctx, precision, sys, split, tasks, order, grid, ctx->timing);
     for (int q = 0; q < nsys; q++) {
       double sum = 0.0, comp = 0.0;
       for (int64_t t = begin[q]; t < begin[q + 1]; t++)
@@ -314,7 +364,10 @@ int atrip_hip_batch_run(atrip_hip_ctx *ctx, const int *systems, int nsys,
       energies[q] = -(sum + comp);
     }
     if (ctx->timing) {
-      KernelStat &k = ctx->stats["batched"];
+      KernelStat &k = ctx->stats[precision == ATRIP_FP64 ? "batched"
+                                 : precision == ATRIP_SPLIT_FP32
+                                     ? "batched_mixed"
+                                     : "batched_fp32"];
       k.launches++;
       k.ms += r.ms;
       k.flop += flop;
@@ -323,29 +376,31 @@ int atrip_hip_batch_run(atrip_hip_ctx *ctx, const int *systems, int nsys,
 }
 
 int atrip_hip_batch_tune(atrip_hip_ctx *ctx, const int *systems, int nsys,
-                         int mode) {
+                         int precision, int mode) {
   return guarded(ctx, [&] {
     if (nsys < 0 || (nsys > 0 && !systems)) throw Fail{"bad system list"};
     if (mode != 0 && mode != 1) throw Fail{"mode must be 0 or 1"};
+    check_precision(precision);
     check(hipSetDevice(ctx->device), "hipSetDevice");
-    const int grid = ctx->batch_grid ? ctx->batch_grid : default_grid(ctx);
+    const int grid =
+        ctx->batch_grid ? ctx->batch_grid : default_grid(ctx, precision);
     // one representative system per shape
-    std::vector<std::pair<std::pair<int64_t, int64_t>, int>> reps;
+    std::vector<std::pair<std::array<int64_t, 3>, int>> reps;
     for (int q = 0; q < nsys; q++) {
       const DevSystem &s = get_sys(ctx, systems[q]);
       if (s.No > MAX_NO)
         throw Fail{"batched form needs No <= 64 (use atrip_hip_run)"};
-      const auto key = shape_key(s);
+      const auto key = shape_key(s, precision);
       bool seen = false;
       for (auto &r : reps) seen |= r.first == key;
       if (!seen) reps.push_back({key, systems[q]});
     }
     bool warm = false;
     for (auto &rep : reps) {
-      const DevSystem &s = get_sys(ctx, rep.second);
+      DevSystem &s = get_sys(ctx, rep.second);
       if (mode == 0) {
         ctx->btune[rep.first] =
-            model_pick(s.No, s.Nv, ctx->batch_scratch_mb, grid);
+            model_pick(s.No, s.Nv, ctx->batch_scratch_mb, grid, precision);
         continue;
       }
       // candidates: the six best by the model
@@ -353,7 +408,8 @@ int atrip_hip_batch_tune(atrip_hip_ctx *ctx, const int *systems, int nsys,
       for (int nb : NB_CAND) {
         if (!nb_fits(s.No, nb, ctx->batch_scratch_mb, grid)) continue;
         for (int t = 0; t < N_BTILES; t++)
-          cand.push_back({model_sec_tuple(s.No, s.Nv, nb, t), {nb, t}});
+          cand.push_back(
+              {model_sec_tuple(s.No, s.Nv, nb, t, precision), {nb, t}});
       }
       std::stable_sort(cand.begin(), cand.end(),
                        [](const auto &x, const auto &y) { return x.first < y.first; });
@@ -376,12 +432,14 @@ int atrip_hip_batch_tune(atrip_hip_ctx *ctx, const int *systems, int nsys,
         }
         std::vector<int32_t> order(tasks.size());
         std::iota(order.begin(), order.end(), 0);
-        const std::vector<BSys> sys{make_bsys(s, p)};
+        const std::vector<BSys> sys{make_bsys(ctx, s, p, precision)};
+        const std::vector<BSplit> split{make_bsplit(s)};
         if (!warm) {
-          run_tasks(ctx, sys, tasks, order, grid, false);
+          run_tasks(ctx, precision, sys, split, tasks, order, grid, false);
           warm = true;
         }
-        const LaunchOut r = run_tasks(ctx, sys, tasks, order, grid, true);
+        const LaunchOut r =
+            run_tasks(ctx, precision, sys, split, tasks, order, grid, true);
         const double rate = tuples / std::max(1e-6, (double)r.ms);
         if (rate > best_rate) {
           best_rate = rate;

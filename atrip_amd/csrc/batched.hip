@@ -27,6 +27,8 @@ namespace atrip_amd {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -162,6 +164,154 @@ __device__ __forceinline__ void wave_tile(const BSys &S, int cl, int al,
   }
 }
 
+// The same wave tile with FP32 products (mixed forms): hi*hi on
+// v_mfma_f32_16x16x4f32 and, when CORR, hi*lo + lo*hi in one
+// v_mfma_f32_16x16x32_bf16 per 16-deep chunk (lanes 0..31 carry hi_a and
+// lo_b, lanes 32..63 lo_a and hi_b), FP32 accumulation; the tile is widened
+// to FP64 when it is added into W.
+template <int FM, int FN, bool CORR>
+__device__ __forceinline__ void wave_tile_split(const BSys &S, const BSplit &X,
+                                                int cl, int al, int be, int ga,
+                                                int o1, int o2, int mt, int nt,
+                                                double *Gwin, bool add,
+                                                int lane) {
+  const int nb = S.nb;
+  const int64_t No = S.No, Nv = S.Nv, N2 = No * No, Kv = S.Kv, Ko = S.Ko;
+  const int kk = lane >> 4, l16 = lane & 15;
+  int32_t ca[FN], cb[FN], cc[FN], cslot[FN], crr[FN];
+  bool cvalid[FN];
+  bool any = false;
+#pragma unroll
+  for (int g = 0; g < FN; g++) {
+    const int64_t n = (int64_t)nt * 16 * FN + g * 16 + l16;
+    const int t = (int)(n / No);
+    crr[g] = (int32_t)(n % No);
+    int64_t a, b, c, slot;
+    class_tuple(cl, al, be, ga, nb, o1, o2, t, a, b, c, slot);
+    const bool ok = t < nb && tuple_ok(Nv, a, b, c);
+    cvalid[g] = ok;
+    any |= ok;
+    if (!ok) { a = 0; b = 0; c = 0; crr[g] = 0; slot = 0; }
+    ca[g] = (int32_t)a; cb[g] = (int32_t)b; cc[g] = (int32_t)c;
+    cslot[g] = (int32_t)slot;
+  }
+  if (!__any(any)) return;
+
+  int64_t x1, x2;
+  class_pair(cl, al, be, ga, nb, o1, o2, x1, x2);
+  if (x1 >= Nv) x1 = Nv - 1;
+  if (x2 >= Nv) x2 = Nv - 1;
+  int32_t rowS[FM], rowW[FM];
+#pragma unroll
+  for (int f = 0; f < FM; f++) {
+    int64_t m = (int64_t)mt * 16 * FM + f * 16 + l16;
+    if (m >= N2) m = N2 - 1;
+    rowS[f] = (int32_t)m;
+    rowW[f] = (int32_t)(m / No + No * (m % No));
+  }
+  const bool lower = kk < 2;
+  const int coff = 8 * (kk & 1);
+
+  f4 acc[FM][FN];
+#pragma unroll
+  for (int f = 0; f < FM; f++)
+#pragma unroll
+    for (int g = 0; g < FN; g++) acc[f][g] = f4{0.f, 0.f, 0.f, 0.f};
+  const float *aph[FM], *bph[FN];
+  const __bf16 *apc[FM], *bpc[FN];
+  f4 ah0[FM], bh0[FN], ah1[FM], bh1[FN];
+  bf8 ac0[FM], bc0[FN], ac1[FM], bc1[FN];
+#define LOAD(AH, AC, BH, BC, KOFF)                                          \
+  {                                                                         \
+    _Pragma("unroll") for (int f = 0; f < FM; f++) {                        \
+      AH[f] = *(const f4 *)(aph[f] + (KOFF));                               \
+      if constexpr (CORR) AC[f] = *(const bf8 *)(apc[f] + (KOFF));         \
+    }                                                                       \
+    _Pragma("unroll") for (int g = 0; g < FN; g++) {                        \
+      BH[g] = *(const f4 *)(bph[g] + (KOFF));                               \
+      if constexpr (CORR) BC[g] = *(const bf8 *)(bpc[g] + (KOFF));         \
+    }                                                                       \
+    __builtin_amdgcn_sched_barrier(0);                                      \
+  }
+#define COMPUTE(AH, AC, BH, BC)                                             \
+  {                                                                         \
+    _Pragma("unroll") for (int s = 0; s < 4; s++)                           \
+        _Pragma("unroll") for (int f = 0; f < FM; f++)                      \
+            _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =      \
+                __builtin_amdgcn_mfma_f32_16x16x4f32(AH[f][s], BH[g][s],    \
+                                                     acc[f][g], 0, 0, 0);   \
+    if constexpr (CORR)                                                     \
+      _Pragma("unroll") for (int f = 0; f < FM; f++)                        \
+          _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =        \
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(AC[f], BC[g],         \
+                                                      acc[f][g], 0, 0, 0);  \
+  }
+#pragma unroll 1
+  for (int sg = 0; sg < 4; sg++) {
+    const int64_t K_ = sg < 2 ? Kv : Ko;
+    const int64_t xa = (sg == 0 || sg == 3) ? x1 : x2;
+    const int64_t abase = xa * N2 * K_;
+    const float *Ah = (sg < 2 ? X.T3h : X.H3h) + abase;
+    const __bf16 *Ac =
+        (const __bf16 *)(lower ? (sg < 2 ? X.T3b : X.H3b)
+                               : (sg < 2 ? X.T3l : X.H3l)) +
+        abase;
+#pragma unroll
+    for (int f = 0; f < FM; f++) {
+      const int64_t r = (int64_t)((sg & 1) ? rowW[f] : rowS[f]) * K_;
+      aph[f] = Ah + r + 4 * kk;
+      apc[f] = Ac + r + coff;
+    }
+#pragma unroll
+    for (int g = 0; g < FN; g++) {
+      int64_t u, v;
+      const int wh = b_pair(cl, sg, ca[g], cb[g], cc[g], u, v);
+      const int64_t c = ((u + Nv * v) * No + crr[g]) * K_;
+      const float *Bh = wh == 0 ? X.V3h : wh == 1 ? X.T2Nh : X.T2TNh;
+      const uint16_t *Bc = lower ? (wh == 0 ? X.V3l : wh == 1 ? X.T2Nl : X.T2TNl)
+                                 : (wh == 0 ? X.V3b : wh == 1 ? X.T2Nb : X.T2TNb);
+      bph[g] = Bh + c + 4 * kk;
+      bpc[g] = (const __bf16 *)Bc + c + coff;
+    }
+    const int64_t last = K_ - KCHUNK;
+    LOAD(ah0, ac0, bh0, bc0, 0);
+    for (int64_t k0 = 0;; k0 += 2 * KCHUNK) {
+      LOAD(ah1, ac1, bh1, bc1, k0 + KCHUNK < last ? k0 + KCHUNK : last);
+      COMPUTE(ah0, ac0, bh0, bc0);
+      if (k0 + KCHUNK > last) break;
+      LOAD(ah0, ac0, bh0, bc0, k0 + 2 * KCHUNK < last ? k0 + 2 * KCHUNK : last);
+      COMPUTE(ah1, ac1, bh1, bc1);
+      if (k0 + 2 * KCHUNK > last) break;
+    }
+  }
+#undef LOAD
+#undef COMPUTE
+
+  // 16x16 FP32-output MFMAs: D[i][j] in lane j + 16*(i/4), register i%4
+  const int no = (int)No, n2 = (int)N2;
+#pragma unroll
+  for (int g = 0; g < FN; g++) {
+    if (!cvalid[g]) continue;
+    double *Gs = Gwin + cslot[g] * (n2 * no);
+    const int rr = crr[g];
+#pragma unroll
+    for (int f = 0; f < FM; f++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = mt * 16 * FM + f * 16 + 4 * kk + r;
+        if (m >= n2) continue;
+        const int p = m % no, q = m / no;
+        const int off = cl == 0   ? p + no * (q + no * rr)
+                        : cl == 1 ? p + no * (rr + no * q)
+                                  : rr + no * (p + no * q);
+        double *dst = Gs + off;
+        const double v = (double)acc[f][g][r];
+        *dst = add ? *dst + v : v;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+  }
+}
+
 // Energy of tuple (a,b,c) from its W (order i + No j + No^2 k), by one
 // wave; tA/tB/tC hold Tai at (a,x), (b,x), (c,x).
 __device__ __forceinline__ double tuple_energy(const BSys &S, const double *W, int64_t a,
@@ -235,6 +385,7 @@ __device__ __forceinline__ T uniform_load(const T *p) {
   return v;
 }
 
+template <int PREC>
 __global__ __launch_bounds__(256) void batched_kernel(BatchedLaunch P) {
   __shared__ int task_sh;
   __shared__ double etup[512];
@@ -252,6 +403,8 @@ __global__ __launch_bounds__(256) void batched_kernel(BatchedLaunch P) {
     const int tix = uniform_load(P.order + q);
     const BTask T = uniform_load(P.tasks + tix);
     const BSys S = uniform_load(P.sys + T.sys);
+    BSplit X{};
+    if constexpr (PREC != ATRIP_FP64) X = uniform_load(P.split + T.sys);
     const int nb = S.nb, nb3 = nb * nb * nb;
     const int64_t No = S.No, N2 = No * No;
     const int fm = bfm(S.tile), fn = bfn(S.tile);
@@ -267,10 +420,14 @@ __global__ __launch_bounds__(256) void batched_kernel(BatchedLaunch P) {
         const int pair = u / (ntiles * mtiles);
         const int o1 = pair / nb, o2 = pair % nb;
         switch (S.tile) {
-#define TILE_CASE(V)                                                         \
-  case V:                                                                    \
-    wave_tile<bfm(V), bfn(V)>(S, cl, T.al, T.be, T.ga, o1, o2, mt, nt, Gwin, \
-                              cl > 0, lane);                                 \
+#define TILE_CASE(V)                                                       \
+  case V:                                                                  \
+    if constexpr (PREC == ATRIP_FP64)                                      \
+      wave_tile<bfm(V), bfn(V)>(S, cl, T.al, T.be, T.ga, o1, o2, mt, nt,   \
+                                Gwin, cl > 0, lane);                       \
+    else                                                                   \
+      wave_tile_split<bfm(V), bfn(V), PREC == ATRIP_SPLIT_FP32>(           \
+          S, X, cl, T.al, T.be, T.ga, o1, o2, mt, nt, Gwin, cl > 0, lane); \
     break;
           TILE_CASE(0)
           TILE_CASE(1)
@@ -335,15 +492,33 @@ __global__ __launch_bounds__(256) void batched_kernel(BatchedLaunch P) {
 
 hipError_t launch_batched(const BatchedLaunch &L, hipStream_t st) {
   if (L.ntasks <= 0 || L.grid <= 0) return hipSuccess;
-  hipLaunchKernelGGL(batched_kernel, dim3(L.grid), dim3(256), 0, st, L);
+  if (L.precision == ATRIP_FP64)
+    hipLaunchKernelGGL(batched_kernel<ATRIP_FP64>, dim3(L.grid), dim3(256), 0,
+                       st, L);
+  else if (L.precision == ATRIP_SPLIT_FP32)
+    hipLaunchKernelGGL(batched_kernel<ATRIP_SPLIT_FP32>, dim3(L.grid),
+                       dim3(256), 0, st, L);
+  else if (L.precision == ATRIP_FP32)
+    hipLaunchKernelGGL(batched_kernel<ATRIP_FP32>, dim3(L.grid), dim3(256), 0,
+                       st, L);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-int batched_blocks_per_cu() {
+int batched_blocks_per_cu(int precision) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batched_kernel, 256,
-                                                   0) != hipSuccess)
-    return 1;
+  hipError_t e;
+  if (precision == ATRIP_SPLIT_FP32)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &n, batched_kernel<ATRIP_SPLIT_FP32>, 256, 0);
+  else if (precision == ATRIP_FP32)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &n, batched_kernel<ATRIP_FP32>, 256, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &n, batched_kernel<ATRIP_FP64>, 256, 0);
+  if (e != hipSuccess) return 1;
   return n > 0 ? n : 1;
 }
 

@@ -40,8 +40,12 @@ int guarded(atrip_hip_ctx *ctx, F &&f) {
 // Resident system of a handle; throws Fail for an unknown handle.
 DevSystem &get_sys(atrip_hip_ctx *ctx, int sys);
 
-// Device bytes of the resident arrays of a system (what admission charges).
-uint64_t system_bytes(int64_t No, int64_t Nv, bool with_j);
+// Split-FP32 copies of a system's doubles operands (made once).
+void ensure_split(atrip_hip_ctx *ctx, DevSystem &s);
+
+// Device bytes of the resident arrays of a system uploaded with operand
+// store mode `store` (what admission charges).
+uint64_t system_bytes(int64_t No, int64_t Nv, bool with_j, int store);
 
 // Number of tuples a <= b <= c < Nv, not all equal.
 inline uint64_t n_tuples(uint64_t Nv) {

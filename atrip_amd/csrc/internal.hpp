@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <map>
 #include <string>
@@ -135,8 +136,9 @@ struct atrip_hip_ctx {
   size_t e_slot_bytes = 0;
   double *partials = nullptr;  // per-batch (sum, comp) x 2
   size_t partials_bytes = 0;
-  // batched form: tuned plans keyed by (No, padded Nv), workgroup scratch
-  std::map<std::pair<int64_t, int64_t>, atrip_amd::BatchTune> btune;
+  // batched form: tuned plans keyed by (No, padded Nv, precision),
+  // workgroup scratch
+  std::map<std::array<int64_t, 3>, atrip_amd::BatchTune> btune;
   double *bscratch = nullptr;
   size_t bscratch_bytes = 0;
   int64_t batch_scratch_mb = 8192;
@@ -219,6 +221,14 @@ constexpr __host__ __device__ inline int bfn(int v) {
   return t[v];
 }
 
+// Split-FP32 operands of a system in a batched launch (h = FP32 values,
+// b = bf16(h), l = bf16(x - h); b and l unused by the FP32-only form).
+struct BSplit {
+  const float *T3h, *V3h, *H3h, *T2Nh, *T2TNh;
+  const uint16_t *T3b, *V3b, *H3b, *T2Nb, *T2TNb;
+  const uint16_t *T3l, *V3l, *H3l, *T2Nl, *T2TNl;
+};
+
 // One work item: a window (block triple of edge nb) of one system.
 struct BTask {
   int32_t sys, al, be, ga;
@@ -226,7 +236,9 @@ struct BTask {
 };
 
 struct BatchedLaunch {
+  int precision;            // ATRIP_FP64, ATRIP_SPLIT_FP32 or ATRIP_FP32
   const BSys *sys;          // device
+  const BSplit *split;      // device, one per system (mixed forms only)
   const BTask *tasks;       // device, canonical order (system-major)
   const int32_t *order;     // device, dispatch order (a permutation)
   int ntasks;
@@ -242,8 +254,8 @@ struct BatchedLaunch {
 // until none is left; doubles, singles and energy of a window stay in the
 // workgroup.
 hipError_t launch_batched(const BatchedLaunch &L, hipStream_t st);
-// Workgroups per CU the persistent kernel can hold (for the grid size).
-int batched_blocks_per_cu();
+// Workgroups per CU the persistent kernel of a precision can hold.
+int batched_blocks_per_cu(int precision);
 // Canonical (lexicographic) index of tuple (a,b,c), a<=b<=c, not all equal.
 __host__ __device__ inline uint64_t tuple_index(const uint64_t *aprefix,
                                                 uint64_t Nv, uint64_t a,

@@ -63,6 +63,7 @@ struct atrip_hip_server {
   int max_active = 64;
   double iter_ms = 100.0;
   int policy = 0;  // 0 shortest remaining first, 1 equal share
+  int precision = ATRIP_FP64;
   Clock::time_point t0 = Clock::now();
   std::map<int64_t, Request> reqs;
   std::deque<int64_t> queue;
@@ -105,7 +106,11 @@ void admit(atrip_hip_server *srv, int grid) {
       srv->queue.pop_front();
       continue;
     }
-    r.bytes = system_bytes(r.No, r.Nv, false);
+    // served systems keep only the operand form the server computes in
+    const int store = srv->precision == ATRIP_FP64         ? 0
+                      : srv->precision == ATRIP_SPLIT_FP32 ? 1
+                                                           : 2;
+    r.bytes = system_bytes(r.No, r.Nv, false, store);
     if (r.bytes > srv->budget) {
       fail_request(srv, r, "system larger than the HBM budget");
       srv->queue.pop_front();
@@ -113,13 +118,13 @@ void admit(atrip_hip_server *srv, int grid) {
     }
     if (srv->resident + r.bytes > srv->budget) break;  // wait for memory
     int sys = 0;
-    const int store = ctx->operand_store;
-    ctx->operand_store = 0;  // served systems keep FP64 operands
+    const int store_before = ctx->operand_store;
+    ctx->operand_store = store;
     const int rc =
         r.synthetic
             ? atrip_hip_upload_synthetic(ctx, r.No, r.Nv, r.seed, 0, &sys)
             : atrip_hip_upload(ctx, &r.desc, &sys);
-    ctx->operand_store = store;
+    ctx->operand_store = store_before;
     if (rc != 0) {
       // out of device memory with other requests resident: wait for them
       if (!srv->active.empty() &&
@@ -135,7 +140,7 @@ void admit(atrip_hip_server *srv, int grid) {
     r.t_admit = srv->now();
     srv->resident += r.bytes;
     srv->peak_resident = std::max(srv->peak_resident, srv->resident);
-    r.plan = plan_for(ctx, get_sys(ctx, sys), grid);
+    r.plan = plan_for(ctx, get_sys(ctx, sys), grid, srv->precision);
     r.wins = windows_of(r.Nv, r.plan.nb);
     r.cursor = 0;
     if (r.wins.empty())
@@ -150,7 +155,8 @@ void admit(atrip_hip_server *srv, int grid) {
 int step(atrip_hip_server *srv) {
   atrip_hip_ctx *ctx = srv->ctx;
   check(hipSetDevice(ctx->device), "hipSetDevice");
-  const int grid = ctx->batch_grid ? ctx->batch_grid : default_grid(ctx);
+  const int grid = ctx->batch_grid ? ctx->batch_grid
+                                   : default_grid(ctx, srv->precision);
   const size_t done_before = srv->reqs.size() - srv->queue.size() -
                              srv->active.size();
   admit(srv, grid);
@@ -170,6 +176,7 @@ int step(atrip_hip_server *srv) {
   const double target = srv->iter_ms * 1e-3 * grid;
   double left = target;
   std::vector<BSys> sys;
+  std::vector<BSplit> split;
   std::vector<BTask> tasks;
   std::vector<double> cost;
   std::vector<std::pair<int64_t, size_t>> took;  // request, windows taken
@@ -182,7 +189,9 @@ int step(atrip_hip_server *srv) {
     size_t k = (size_t)std::ceil(std::max(want, 0.0) / sw);
     k = std::max<size_t>(1, std::min(k, r.wins.size() - r.cursor));
     const int si = (int)sys.size();
-    sys.push_back(make_bsys(get_sys(ctx, r.sys), r.plan));
+    DevSystem &ds = get_sys(ctx, r.sys);
+    sys.push_back(make_bsys(ctx, ds, r.plan, srv->precision));
+    split.push_back(make_bsplit(ds));
     const double nb3 = (double)r.plan.nb * r.plan.nb * r.plan.nb;
     for (size_t q = 0; q < k; q++) {
       const Win &w = r.wins[r.cursor + q];
@@ -196,7 +205,8 @@ int step(atrip_hip_server *srv) {
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(),
                    [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
-  const LaunchOut out = run_tasks(ctx, sys, tasks, order, grid, true);
+  const LaunchOut out =
+      run_tasks(ctx, srv->precision, sys, split, tasks, order, grid, true);
   srv->busy_ms += out.ms;
   srv->launches++;
 
@@ -236,10 +246,11 @@ int64_t submit(atrip_hip_server *srv, Request r, double arrival_ms) {
 extern "C" {
 
 int atrip_hip_serve_create(atrip_hip_ctx *ctx, uint64_t hbm_budget_bytes,
-                           int max_active, double iter_ms,
+                           int max_active, double iter_ms, int precision,
                            atrip_hip_server **out) {
   return guarded(ctx, [&] {
     if (!out) throw Fail{"null argument"};
+    check_precision(precision);
     if (hbm_budget_bytes == 0) throw Fail{"HBM budget must be positive"};
     if (max_active < 1) throw Fail{"max_active must be positive"};
     if (!(iter_ms > 0)) throw Fail{"iter_ms must be positive"};
@@ -248,6 +259,7 @@ int atrip_hip_serve_create(atrip_hip_ctx *ctx, uint64_t hbm_budget_bytes,
     s->budget = hbm_budget_bytes;
     s->max_active = max_active;
     s->iter_ms = iter_ms;
+    s->precision = precision;
     *out = s;
   });
 }

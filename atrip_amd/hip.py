@@ -87,16 +87,17 @@ def load():
     L.atrip_hip_device_count.argtypes = []
     L.atrip_hip_device_count.restype = C.c_int
     ip = C.POINTER(C.c_int)
-    L.atrip_hip_batch_run.argtypes = [vp, ip, C.c_int, C.POINTER(C.c_double)]
-    L.atrip_hip_batch_tune.argtypes = [vp, ip, C.c_int, C.c_int]
-    L.atrip_hip_batch_plan.argtypes = [vp, C.c_int, ip, ip, ip, ip,
+    L.atrip_hip_batch_run.argtypes = [vp, ip, C.c_int, C.c_int,
+                                      C.POINTER(C.c_double)]
+    L.atrip_hip_batch_tune.argtypes = [vp, ip, C.c_int, C.c_int, C.c_int]
+    L.atrip_hip_batch_plan.argtypes = [vp, C.c_int, C.c_int, ip, ip, ip, ip,
                                        C.POINTER(C.c_double)]
-    L.atrip_hip_batch_model.argtypes = [C.c_int64, C.c_int64, C.c_int64,
-                                        C.c_int, ip, ip, ip]
+    L.atrip_hip_batch_model.argtypes = [C.c_int64, C.c_int64, C.c_int,
+                                        C.c_int64, C.c_int, ip, ip, ip]
     dp = C.POINTER(C.c_double)
     u64p = C.POINTER(C.c_uint64)
     L.atrip_hip_serve_create.argtypes = [vp, C.c_uint64, C.c_int, C.c_double,
-                                         C.POINTER(vp)]
+                                         C.c_int, C.POINTER(vp)]
     L.atrip_hip_serve_destroy.argtypes = [vp]
     L.atrip_hip_serve_set_policy.argtypes = [vp, C.c_int]
     L.atrip_hip_serve_clock_ms.argtypes = [vp]
@@ -116,11 +117,12 @@ def device_count() -> int:
     return int(load().atrip_hip_device_count())
 
 
-def batch_model(no: int, nv: int, scratch_mb: int = 8192, grid: int = 256):
+def batch_model(no: int, nv: int, scratch_mb: int = 8192, grid: int = 256,
+                precision="fp64"):
     """Cost-model plan (window edge, fm, fn) of the batched form; host only."""
     nb, fm, fn = C.c_int(), C.c_int(), C.c_int()
-    rc = load().atrip_hip_batch_model(no, nv, scratch_mb, grid, C.byref(nb),
-                                      C.byref(fm), C.byref(fn))
+    rc = load().atrip_hip_batch_model(no, nv, PRECISIONS[precision], scratch_mb,
+                                      grid, C.byref(nb), C.byref(fm), C.byref(fn))
     if rc != 0:
         raise HipError(f"atrip_hip_batch_model({no}, {nv}) failed ({rc})")
     return nb.value, fm.value, fn.value
@@ -229,25 +231,26 @@ class Context:
         return (e[:count], ect[:count]) if with_ct else e[:count]
 
     # -- batched form ----------------------------------------------------
-    def batch_run(self, systems) -> np.ndarray:
+    def batch_run(self, systems, precision="fp64") -> np.ndarray:
         """E(T) of every listed resident system, all in one persistent launch."""
         arr = (C.c_int * max(len(systems), 1))(*systems)
         e = np.zeros(max(len(systems), 1))
-        self._check(self._L.atrip_hip_batch_run(self._h, arr, len(systems), _p(e)),
-                    "batch_run")
+        self._check(self._L.atrip_hip_batch_run(
+            self._h, arr, len(systems), PRECISIONS[precision], _p(e)), "batch_run")
         return e[:len(systems)]
 
-    def batch_tune(self, systems, measure=True):
+    def batch_tune(self, systems, measure=True, precision="fp64"):
         arr = (C.c_int * max(len(systems), 1))(*systems)
-        self._check(self._L.atrip_hip_batch_tune(self._h, arr, len(systems),
-                                                 1 if measure else 0), "batch_tune")
+        self._check(self._L.atrip_hip_batch_tune(
+            self._h, arr, len(systems), PRECISIONS[precision],
+            1 if measure else 0), "batch_tune")
 
-    def batch_plan(self, sys_: int) -> dict:
+    def batch_plan(self, sys_: int, precision="fp64") -> dict:
         nb, fm, fn, meas = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         sec = C.c_double()
         self._check(self._L.atrip_hip_batch_plan(
-            self._h, sys_, C.byref(nb), C.byref(fm), C.byref(fn), C.byref(meas),
-            C.byref(sec)), "batch_plan")
+            self._h, sys_, PRECISIONS[precision], C.byref(nb), C.byref(fm),
+            C.byref(fn), C.byref(meas), C.byref(sec)), "batch_plan")
         return dict(nb=nb.value, fm=fm.value, fn=fn.value,
                     measured=bool(meas.value), sec_window=sec.value)
 

@@ -22,13 +22,14 @@ class Server:
     """A serving loop on a hip.Context (which must outlive it)."""
 
     def __init__(self, ctx: hip.Context, hbm_budget_gb: float = 200.0,
-                 max_active: int = 64, iter_ms: float = 100.0, policy: int = 0):
+                 max_active: int = 64, iter_ms: float = 100.0, policy: int = 0,
+                 precision: str = "fp64"):
         self.ctx = ctx
         self._L = ctx._L
         self._h = C.c_void_p()
         ctx._check(self._L.atrip_hip_serve_create(
             ctx._h, int(hbm_budget_gb * 1e9), max_active, iter_ms,
-            C.byref(self._h)), "serve_create")
+            hip.PRECISIONS[precision], C.byref(self._h)), "serve_create")
         if policy:
             ctx._check(self._L.atrip_hip_serve_set_policy(self._h, policy),
                        "serve_set_policy")
@@ -116,12 +117,13 @@ class Server:
 def run_poisson(ctx: hip.Context, n_requests: int, rate: float = 20.0,
                 seed: int = 0, hbm_budget_gb: float = 200.0,
                 max_active: int = 64, iter_ms: float = 100.0, policy: int = 0,
-                no=(8, 40), nv=(40, 320), log=None) -> dict:
+                no=(8, 40), nv=(40, 320), precision: str = "fp64",
+                log=None) -> dict:
     """Replay a Poisson arrival window of n_requests against the server's
     clock (requests are submitted when their arrival time has passed, stamped
     with that arrival time) and report latency percentiles and throughput."""
     reqs = workloads.serve_requests(n_requests, rate=rate, seed=seed, no=no, nv=nv)
-    with Server(ctx, hbm_budget_gb, max_active, iter_ms, policy) as srv:
+    with Server(ctx, hbm_budget_gb, max_active, iter_ms, policy, precision) as srv:
         t0 = srv.clock_ms()
         ids = []
         nxt = 0
@@ -153,7 +155,7 @@ def run_poisson(ctx: hip.Context, n_requests: int, rate: float = 20.0,
     span_s = (t_end - t0) * 1e-3
     flop = sum(workloads.doubles_flop(r[1], r[2]) for r in reqs)
     return dict(
-        requests=len(reqs), rate_req_s=rate, p50_ms=float(np.percentile(lat, 50)),
+        precision=precision, requests=len(reqs), rate_req_s=rate, p50_ms=float(np.percentile(lat, 50)),
         p99_ms=float(np.percentile(lat, 99)), mean_ms=float(lat.mean()),
         max_ms=float(lat.max()), p50_queue_ms=float(np.percentile(queue, 50)),
         span_s=span_s, tuples=int(st["tuples_done"]),

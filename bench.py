@@ -116,6 +116,101 @@ def cpu_baseline(no: int, nv: int, seed: int, budget_s: float, cores: int):
     }
 
 
+def bench_batched(ctx, n_systems: int, seed: int):
+    """Batched form: n seeded small systems, autotuned, one persistent launch."""
+    from atrip_amd import workloads as W
+
+    shapes = W.batch_shapes(n_systems, seed=seed)
+    t0 = time.time()
+    systems = [ctx.upload_synthetic(no, nv, s) for no, nv, s in shapes]
+    up_s = time.time() - t0
+    t0 = time.time()
+    ctx.batch_tune(systems, measure=True)
+    tune_s = time.time() - t0
+    ctx.batch_run(systems)  # warm
+    ctx.reset_stats()
+    ctx.set_option("timing", 1)
+    t0 = time.perf_counter()
+    e = ctx.batch_run(systems)
+    wall = time.perf_counter() - t0
+    ctx.set_option("timing", 0)
+    k = ctx.kernel_stats("batched")
+    tuples = sum(W.n_tuples(nv) for _, nv, _ in shapes)
+    flop = sum(W.doubles_flop(no, nv) for no, nv, _ in shapes)
+    for s_ in systems:
+        ctx.release(s_)
+    return {
+        "workload": (f"{n_systems} systems, No~U{{4..16}}, Nv~U{{20..120}}, "
+                     f"shape seed {seed}, tensor seeds 1000+s; FP64"),
+        "tuples": tuples,
+        "value": tuples / wall, "unit": "tuples/s",
+        "tflops": flop / wall / 1e12,
+        "kernel_ms": k.total_ms,
+        "kernel_tflops": flop / (k.total_ms * 1e-3) / 1e12 if k.total_ms else None,
+        "kernel_frac_of_fp64_peak": (flop / (k.total_ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]) if k.total_ms else None,
+        "wall_s": wall, "upload_s": up_s, "tune_s": tune_s,
+        "energy_sum_Eh": float(sum(e)),
+    }
+
+
+def bench_serving(ctx, n_requests: int, rate: float, seed: int):
+    from atrip_amd import serve
+
+    r = serve.run_poisson(ctx, n_requests, rate=rate, seed=seed,
+                          hbm_budget_gb=200.0, iter_ms=100.0)
+    r.pop("energies")
+    r["workload"] = (f"{n_requests} requests, Poisson {rate} req/s, No log-U[8,40], "
+                     f"Nv log-U[40,320], seed {seed}, tensor seeds 5000+r; "
+                     "HBM budget 200 GB; FP64")
+    return r
+
+
+def bench_c3(ctx, no: int, nv: int, seed: int, a0: int, a1: int, window: int):
+    """Plain FP64 vs the mixed forms at No=60 Nv=600 on the tuples whose first
+    index is in [a0, a1) (whole windows), each form on its own resident copy
+    (FP64 operands; split-only; FP32-only)."""
+    first = tuples_before(a0, nv)
+    count = tuples_before(a1, nv) - first
+    flop_tuple = 12.0 * no ** 3 * (no + nv)
+    out = {"workload": (f"(T) No={no} Nv={nv} seed {seed}, tuples with first index "
+                        f"in [{a0},{a1}) ({count})"), "tuples": count}
+    ref = None
+    for prec, store in (("fp64", 0), ("split_fp32", 1), ("fp32", 2)):
+        ctx.set_option("operand_store", store)
+        try:
+            t0 = time.time()
+            s_ = ctx.upload_synthetic(no, nv, seed)
+            up = time.time() - t0
+        finally:
+            ctx.set_option("operand_store", 0)
+        try:
+            ctx.run(s_, prec, first=first, count=min(count, 1))  # warm
+            kname = {"fp64": "doubles", "split_fp32": "doubles_mixed",
+                     "fp32": "doubles_fp32"}[prec]
+            ctx.reset_stats()
+            ctx.set_option("timing", 1)
+            t0 = time.perf_counter()
+            e, _ = ctx.run(s_, prec, first=first, count=count)
+            dt = time.perf_counter() - t0
+            ctx.set_option("timing", 0)
+            k = ctx.kernel_stats(kname)
+            info = ctx.sys_info(s_)
+        finally:
+            ctx.release(s_)
+        ent = {"value": count / dt, "unit": "tuples/s",
+               "tflops": count * flop_tuple / dt / 1e12,
+               "doubles_kernel_tflops": k.flop / (k.total_ms * 1e-3) / 1e12 if k.total_ms else None,
+               "energy_Eh": e, "resident_gb": info["bytes"] / 1e9, "upload_s": up}
+        if ref is None:
+            ref = ent
+        else:
+            ent["speedup_vs_fp64"] = ent["value"] / ref["value"]
+            ent["abs_dE_Eh"] = abs(e - ref["energy_Eh"])
+            ent["rel_dE"] = abs(e - ref["energy_Eh"]) / abs(ref["energy_Eh"])
+        out[prec] = ent
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +231,13 @@ def main(argv=None):
                     help="skip the mixed-precision comparison pass")
     ap.add_argument("--mixed", default="split_fp32", choices=["split_fp32", "fp32"],
                     help="form of the comparison pass")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=256)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the No=60 Nv=600 FP64-vs-mixed sample")
+    ap.add_argument("--c3-a0", type=int, default=400)
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -213,6 +315,14 @@ def main(argv=None):
             "rel_dE": abs(m_sum - e_sum) / abs(e_sum) if e_sum else None,
         }
 
+    ctx.release(sys_)
+    batched = None if args.no_batched else bench_batched(ctx, args.batch_systems, 0)
+    serving = None if args.no_serve else bench_serving(ctx, args.serve_requests, 20.0, 0)
+    c3 = None
+    if not args.no_c3:
+        c3 = bench_c3(ctx, 60, 600, 1, args.c3_a0, args.c3_a0 + args.window,
+                      args.window)
+
     cpu = None
     if not args.no_cpu:
         cores = min(len(os.sched_getaffinity(0)), 16)
@@ -259,6 +369,9 @@ def main(argv=None):
         "energy_Eh": e_sum if full else None,
         "setup_s": gen_s,
         "mixed_precision": mixed,
+        "mixed_c3": c3,
+        "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -134,8 +134,9 @@ int atrip_hip_sys_info(atrip_hip_ctx *ctx, int sys, int64_t *No, int64_t *Nv,
 int atrip_hip_device_count(void);
 
 /*
- * Batched form: the full tuple spaces of nsys resident systems (plain FP64,
- * No <= 64) in one persistent launch; energies[s] receives E(T) of
+ * Batched form: the full tuple spaces of nsys resident systems (No <= 64) in
+ * one persistent launch, in `precision` (enum atrip_precision; the mixed
+ * forms widen each FP32 tile into an FP64 W); energies[s] receives E(T) of
  * systems[s] with the sign of src/atrip/Atrip.cxx:1108-1111.  No
  * counterpart in the reference, which runs one system per atrip::Atrip::run
  * (src/atrip/Atrip.cxx:686-1111); each system's result is that of
@@ -145,7 +146,7 @@ int atrip_hip_device_count(void);
  * shapes not tuned yet use the cost model.
  */
 int atrip_hip_batch_run(atrip_hip_ctx *ctx, const int *systems, int nsys,
-                        double *energies);
+                        int precision, double *energies);
 
 /*
  * Autotuner of the batched form.  mode 0: cost model only; mode 1: for each
@@ -156,17 +157,18 @@ int atrip_hip_batch_run(atrip_hip_ctx *ctx, const int *systems, int nsys,
  * used by atrip_hip_batch_run and the serving loop.
  */
 int atrip_hip_batch_tune(atrip_hip_ctx *ctx, const int *systems, int nsys,
-                         int mode);
+                         int precision, int mode);
 
 /* Plan used for a resident system: window edge, tile variant (wave tile of
  * 16*fm x 16*fn outputs), whether it was measured, and the estimated
  * seconds per full window. */
-int atrip_hip_batch_plan(atrip_hip_ctx *ctx, int sys, int *nb, int *fm,
-                         int *fn, int *measured, double *sec_window);
+int atrip_hip_batch_plan(atrip_hip_ctx *ctx, int sys, int precision, int *nb,
+                         int *fm, int *fn, int *measured, double *sec_window);
 
 /* Cost-model choice for a shape, on the host only (no device needed). */
-int atrip_hip_batch_model(int64_t No, int64_t Nv, int64_t scratch_mb,
-                          int grid, int *nb, int *fm, int *fn);
+int atrip_hip_batch_model(int64_t No, int64_t Nv, int precision,
+                          int64_t scratch_mb, int grid, int *nb, int *fm,
+                          int *fn);
 
 /*
  * Serving form: a request loop with continuous batching over one context.
@@ -175,15 +177,17 @@ int atrip_hip_batch_model(int64_t No, int64_t Nv, int64_t scratch_mb,
  * resident arrays fit hbm_budget_bytes; each atrip_hip_serve_step admits
  * what fits and runs one persistent launch over windows of every active
  * request (shortest remaining work first by default), about iter_ms long,
- * then retires finished requests and frees their memory.  A request's
- * energy does not depend on the mix it was served in.  The server uses its
+ * then retires finished requests and frees their memory.  All requests of
+ * a server compute in its `precision`; the systems keep only that operand
+ * form resident (operand_store 0, 1 or 2).  A request's energy does not
+ * depend on the mix it was served in.  The server uses its
  * context's stream and must be driven from the context's thread; the
  * context must outlive it.  Errors are reported on the context.
  */
 typedef struct atrip_hip_server atrip_hip_server;
 
 int atrip_hip_serve_create(atrip_hip_ctx *ctx, uint64_t hbm_budget_bytes,
-                           int max_active, double iter_ms,
+                           int max_active, double iter_ms, int precision,
                            atrip_hip_server **out);
 int atrip_hip_serve_destroy(atrip_hip_server *srv);
 /* 0: shortest remaining work first (default), 1: equal share per request */

@@ -145,3 +145,21 @@ def test_batched_errors(hip_ctx):
         hip_ctx.release(big)
     with pytest.raises(hip.HipError, match="batch_nb"):
         hip_ctx.set_option("batch_nb", 3)
+
+
+@pytest.mark.parametrize("precision", ["split_fp32", "fp32"])
+def test_batched_mixed_forms(hip_ctx, batch, precision):
+    """The mixed forms in the batched kernel: the same FP32 products in the
+    same K order as the plain mixed kernel, so E(T) agrees with the plain
+    path's mixed run to the FP64 energy tolerance; alone equals mix."""
+    print("batched", precision, flush=True)
+    e = hip_ctx.batch_run(batch, precision)
+    for s, ei, shape in zip(batch, e, SHAPES):
+        ep, _ = hip_ctx.run(s, precision)
+        assert _close(ei, ep, REL_PLAIN), (shape, ei, ep)
+    assert hip_ctx.batch_run(batch[3:4], precision)[0] == e[3]
+    hip_ctx.batch_tune(batch[:4], measure=True, precision=precision)
+    assert hip_ctx.batch_plan(batch[0], precision)["measured"]
+    e2 = hip_ctx.batch_run(batch, precision)
+    for q in range(len(batch)):
+        assert _close(e2[q], e[q], REL_PLAIN)

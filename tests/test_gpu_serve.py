@@ -122,3 +122,31 @@ def test_poisson_driver_reports_percentiles(hip_ctx):
     assert r["requests"] == 12 and len(r["energies"]) == 12
     assert 0 < r["p50_ms"] <= r["p99_ms"] <= r["max_ms"]
     assert r["tuples"] > 0 and r["tuples_per_s"] > 0
+
+
+def test_mixed_precision_serving(hip_ctx):
+    """A split-FP32 server: each request keeps only its split operands, and
+    its energy is the same alone and in the mix and agrees with the plain
+    path's split-FP32 run."""
+    print("serve split_fp32", flush=True)
+    mix = MIX[:4]
+    alone = {}
+    for m in mix:
+        with serve.Server(hip_ctx, hbm_budget_gb=50, precision="split_fp32") as srv:
+            rid = srv.submit_synthetic(*m)
+            alone[m] = srv.drain([rid])[rid]["energy"]
+    with serve.Server(hip_ctx, hbm_budget_gb=50, iter_ms=5.0,
+                      precision="split_fp32") as srv:
+        ids = [srv.submit_synthetic(*mix[0])]
+        for m in mix[1:]:
+            srv.step()
+            ids.append(srv.submit_synthetic(*m))
+        res = srv.drain(ids)
+    for rid, m in zip(ids, mix):
+        assert res[rid]["energy"] == alone[m]
+        s = hip_ctx.upload_synthetic(*m)
+        try:
+            ep, _ = hip_ctx.run(s, "split_fp32")
+        finally:
+            hip_ctx.release(s)
+        assert _close(alone[m], ep, REL_PLAIN)
