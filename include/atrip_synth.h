@@ -23,6 +23,10 @@
  *   Vijka, Vabci, Jijka, Jabci ~ N(0,5e-2).
  * Symmetrisation draws the element whose (a,i) pair index a+Nv*i is not
  * larger than (b,j)'s and mirrors it, so marginals stay unchanged.
+ * Complex systems: the imaginary part of tensor t is tensor t |
+ * ATRIP_SYN_IMAG, with t's distribution and symmetrisation from its own
+ * stream; orbital energies stay real (imaginary part 0), as in the
+ * reference bench's complex mode (bench/main.cxx:224-232).
  */
 #ifndef ATRIP_SYNTH_H
 #define ATRIP_SYNTH_H
@@ -52,7 +56,8 @@ enum atrip_synth_tensor {
   ATRIP_SYN_VIJKA = 5,
   ATRIP_SYN_VABCI = 6,
   ATRIP_SYN_JIJKA = 7,
-  ATRIP_SYN_JABCI = 8
+  ATRIP_SYN_JABCI = 8,
+  ATRIP_SYN_IMAG = 16 /* flag: imaginary part of the tensor */
 };
 
 ATRIP_SYNTH_FN uint64_t atrip_synth_mix64(uint64_t z) {
@@ -187,7 +192,9 @@ ATRIP_SYNTH_FN double atrip_synth_normal(uint64_t s, uint64_t n) {
 ATRIP_SYNTH_FN double atrip_synth_value(uint64_t s, int tensor, uint64_t No,
                                         uint64_t Nv, uint64_t idx) {
   ATRIP_SYNTH_NOCONTRACT
-  switch (tensor) {
+  const int base = tensor & 15;
+  if ((tensor & ATRIP_SYN_IMAG) && base <= ATRIP_SYN_EA) return 0.0;
+  switch (base) {
   case ATRIP_SYN_EI: return fma(1.2, atrip_synth_u01(s, idx), -1.5);
   case ATRIP_SYN_EA: return fma(2.9, atrip_synth_u01(s, idx), 0.1);
   case ATRIP_SYN_TAI: return 1e-3 * atrip_synth_normal(s, idx);
@@ -199,7 +206,7 @@ ATRIP_SYNTH_FN double atrip_synth_value(uint64_t s, int tensor, uint64_t No,
     const uint64_t i = (idx / NvNv) % No, j = idx / (NvNv * No);
     uint64_t key = idx;
     if (a + Nv * i > b + Nv * j) key = b + Nv * a + NvNv * j + NvNv * No * i;
-    const double sd = tensor == ATRIP_SYN_TABIJ ? 1e-2 : 5e-2;
+    const double sd = base == ATRIP_SYN_TABIJ ? 1e-2 : 5e-2;
     return sd * atrip_synth_normal(s, key);
   }
   default: return 5e-2 * atrip_synth_normal(s, idx);

@@ -8,6 +8,7 @@
 #include "atrip_oracle.h"
 
 #include <dlfcn.h>
+#include <complex.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
@@ -631,6 +632,238 @@ int oracle_run(const oracle_input *in, int ijkabc, uint64_t first,
   free(args);
   free(th);
   free(Tsl);
+  free(Tai);
+  return status;
+}
+
+/* ------------------------------------------------------------------ */
+/* Complex (T)                                                         */
+/* ------------------------------------------------------------------ */
+
+typedef double _Complex zdouble;
+
+/* src/atrip/Equations.cxx:620-680 with F = Complex: holes use conj(Vhhh)
+ * (MAYBE_CONJ), particles plain products. */
+static void doubles_z(size_t No, size_t Nv, const zdouble *V[6],
+                      const zdouble *Vh[3], const zdouble *T[3],
+                      const zdouble *T2[3], zdouble *Tijk) {
+  const size_t NoNo = No * No, NoNv = No * Nv;
+  const zdouble *VABph = V[0], *VACph = V[1], *VBCph = V[2], *VBAph = V[3],
+                *VCAph = V[4], *VCBph = V[5];
+  const zdouble *VhhhA = Vh[0], *VhhhB = Vh[1], *VhhhC = Vh[2];
+  const zdouble *TAphh = T[0], *TBphh = T[1], *TCphh = T[2];
+  const zdouble *TABhh = T2[0], *TAChh = T2[1], *TBChh = T2[2];
+  for (size_t k = 0; k < No; k++)
+    for (size_t j = 0; j < No; j++)
+      for (size_t i = 0; i < No; i++) {
+        zdouble w = 0.0;
+        for (size_t L = 0; L < No; L++) {
+          w -= TABhh[L + j * No] * conj(VhhhC[i + k * No + L * NoNo]); /* H1 */
+          w -= TABhh[i + L * No] * conj(VhhhC[j + k * No + L * NoNo]); /* H0 */
+          w -= TAChh[L + k * No] * conj(VhhhB[i + j * No + L * NoNo]); /* H5 */
+          w -= TAChh[i + L * No] * conj(VhhhB[k + j * No + L * NoNo]); /* H3 */
+          w -= TBChh[L + k * No] * conj(VhhhA[j + i * No + L * NoNo]); /* H2 */
+          w -= TBChh[j + L * No] * conj(VhhhA[k + i * No + L * NoNo]); /* H4 */
+        }
+        for (size_t E = 0; E < Nv; E++) {
+          w += TAphh[E + i * Nv + j * NoNv] * VBCph[E + k * Nv]; /* P0 */
+          w += TAphh[E + i * Nv + k * NoNv] * VCBph[E + j * Nv]; /* P3 */
+          w += TCphh[E + k * Nv + i * NoNv] * VABph[E + j * Nv]; /* P5 */
+          w += TCphh[E + k * Nv + j * NoNv] * VBAph[E + i * Nv]; /* P2 */
+          w += TBphh[E + j * Nv + i * NoNv] * VACph[E + k * Nv]; /* P1 */
+          w += TBphh[E + j * Nv + k * NoNv] * VCAph[E + i * Nv]; /* P4 */
+        }
+        Tijk[i + j * No + k * NoNo] = w;
+      }
+}
+
+/* src/atrip/Equations.cxx:101-238 with F = Complex, without the 16-blocking
+ * (the sum order within a tuple is free to differ at the tolerance). */
+static double energy_z(int distinct, zdouble epsabc, size_t No,
+                       const zdouble *epsi, const zdouble *Tijk,
+                       const zdouble *Zijk) {
+  zdouble energy = 0.0;
+  for (size_t k = 0; k < No; k++)
+    for (size_t j = k; j < No; j++)
+      for (size_t i = j; i < No; i++) {
+        const zdouble facjk = j == k ? 0.5 : 1.0, facij = i == j ? 0.5 : 1.0;
+        const zdouble denominator = epsabc - ((epsi[i] + epsi[j]) + epsi[k]);
+        zdouble value;
+        if (distinct == 0) {
+          const zdouble U = Zijk[i + No * j + No * No * k],
+                        V = Zijk[i + No * k + No * No * j],
+                        W = Zijk[j + No * i + No * No * k],
+                        X = Zijk[j + No * k + No * No * i],
+                        Y = Zijk[k + No * i + No * No * j],
+                        Z = Zijk[k + No * j + No * No * i];
+          const zdouble A = conj(Tijk[i + No * j + No * No * k]),
+                        B = conj(Tijk[i + No * k + No * No * j]),
+                        C = conj(Tijk[j + No * i + No * No * k]),
+                        D = conj(Tijk[j + No * k + No * No * i]),
+                        E = conj(Tijk[k + No * i + No * No * j]),
+                        F = conj(Tijk[k + No * j + No * No * i]);
+          const zdouble UXY = U + (X + Y), VWZ = V + (W + Z);
+          const zdouble first = A * U + (B * V + (C * W + (D * X + (E * Y + F * Z))));
+          value = 3.0 * first + ((UXY - 2.0 * VWZ) * (A + (D + E)) +
+                                 (VWZ - 2.0 * UXY) * (B + (C + F)));
+        } else {
+          const zdouble U = Zijk[i + No * j + No * No * k],
+                        V = Zijk[j + No * k + No * No * i],
+                        W = Zijk[k + No * i + No * No * j];
+          const zdouble A = conj(Tijk[i + No * j + No * No * k]),
+                        B = conj(Tijk[j + No * k + No * No * i]),
+                        C = conj(Tijk[k + No * i + No * No * j]);
+          value = 3.0 * ((A * U + B * V) + C * W) - (A + (B + C)) * (U + (V + W));
+        }
+        energy += ((2.0 * value) / denominator) * (facjk * facij);
+      }
+  return creal(energy);
+}
+
+typedef struct {
+  const oracle_input_z *in;
+  const zdouble *Tai;
+  uint64_t lo, hi, first;
+  double *pt, *ptc;
+  int status;
+} worker_z_args;
+
+static void *worker_z(void *p) {
+  worker_z_args *w = (worker_z_args *)p;
+  const oracle_input_z *in = w->in;
+  const size_t No = in->No, Nv = in->Nv, NoNo = No * No, N3 = NoNo * No;
+  const size_t Nv2 = Nv * Nv, Nv3 = Nv2 * Nv;
+  const zdouble *Tabij = (const zdouble *)in->Tabij,
+                *Vabij = (const zdouble *)in->Vabij,
+                *Vijka = (const zdouble *)in->Vijka,
+                *Vabci = (const zdouble *)in->Vabci,
+                *Jijka = (const zdouble *)in->Jijka,
+                *Jabci = (const zdouble *)in->Jabci,
+                *ei = (const zdouble *)in->ei, *ea = (const zdouble *)in->ea;
+  const int have_J = Jijka && Jabci;
+  const size_t nbuf = 4 * N3 + 12 * Nv * No + 3 * Nv * NoNo + 6 * NoNo;
+  zdouble *buf = (zdouble *)malloc(sizeof(zdouble) * nbuf);
+  if (!buf) {
+    w->status = -1;
+    return NULL;
+  }
+  zdouble *Tijk = buf, *Zijk = buf + N3, *TJ = buf + 2 * N3;
+  zdouble *vs = buf + 4 * N3, *vsJ = vs + 6 * Nv * No;
+  zdouble *ts = vsJ + 6 * Nv * No, *hh = ts + 3 * Nv * NoNo;
+  for (uint64_t it = w->lo; it < w->hi; it++) {
+    uint64_t abc[3];
+    oracle_nth_tuple(it, Nv, abc);
+    const size_t a = abc[0], b = abc[1], c = abc[2];
+    const size_t xy[6][2] = {{a, b}, {a, c}, {b, c}, {b, a}, {c, a}, {c, b}};
+    const zdouble *V[6], *VJ[6], *Vh[3], *VhJ[3], *T[3], *T2[3], *V2[3];
+    for (int q = 0; q < 6; q++) {
+      zdouble *d = vs + q * Nv * No, *dJ = vsJ + q * Nv * No;
+      for (size_t i = 0; i < No; i++)
+        for (size_t E = 0; E < Nv; E++) {
+          const size_t src = xy[q][0] + Nv * xy[q][1] + Nv2 * E + Nv3 * i;
+          d[E + Nv * i] = Vabci[src];
+          if (have_J) dJ[E + Nv * i] = Jabci[src];
+        }
+      V[q] = d;
+      VJ[q] = dJ;
+    }
+    for (int q = 0; q < 3; q++) {
+      Vh[q] = Vijka + N3 * abc[q];
+      VhJ[q] = have_J ? Jijka + N3 * abc[q] : NULL;
+      zdouble *t = ts + q * Nv * NoNo;
+      for (size_t r = 0; r < Nv * NoNo; r++) t[r] = Tabij[abc[q] + Nv * r];
+      T[q] = t;
+      zdouble *t2 = hh + q * NoNo, *v2 = hh + 3 * NoNo + q * NoNo;
+      for (size_t j = 0; j < No; j++)
+        for (size_t i = 0; i < No; i++) {
+          const size_t src = xy[q][0] + Nv * xy[q][1] + Nv2 * i + Nv2 * No * j;
+          t2[i + No * j] = Tabij[src];
+          v2[i + No * j] = Vabij[src];
+        }
+      T2[q] = t2;
+      V2[q] = v2;
+    }
+    doubles_z(No, Nv, V, Vh, T, T2, Tijk);
+    memcpy(Zijk, Tijk, N3 * sizeof(zdouble));
+    for (size_t k = 0; k < No; k++)
+      for (size_t i = 0; i < No; i++)
+        for (size_t j = 0; j < No; j++) {
+          const size_t ijk = i + j * No + k * NoNo;
+          Zijk[ijk] += w->Tai[a + i * Nv] * V2[2][j + k * No];
+          Zijk[ijk] += w->Tai[b + j * Nv] * V2[1][i + k * No];
+          Zijk[ijk] += w->Tai[c + k * Nv] * V2[0][i + j * No];
+        }
+    int distinct = 0;
+    if (a == b) distinct++;
+    if (b == c) distinct--;
+    const zdouble epsabc = (ea[a] + ea[b]) + ea[c];
+    const double e = energy_z(distinct, epsabc, No, ei, Tijk, Zijk);
+    double ect = e;
+    if (have_J) {
+      doubles_z(No, Nv, VJ, VhJ, T, T2, TJ);
+      ect = energy_z(distinct, epsabc, No, ei, TJ, Zijk);
+    }
+    w->pt[it - w->first] = e;
+    w->ptc[it - w->first] = ect;
+  }
+  free(buf);
+  w->status = 0;
+  return NULL;
+}
+
+int oracle_run_z(const oracle_input_z *in, int ijkabc, uint64_t first,
+                 uint64_t count, int nthreads, double *energy,
+                 double *ct_energy, double *per_tuple, double *per_tuple_ct) {
+  const size_t No = in->No, Nv = in->Nv;
+  const uint64_t n = oracle_n_tuples(Nv);
+  if (first > n) first = n;
+  if (count > n - first) count = n - first;
+  if (nthreads < 1) nthreads = 1;
+  zdouble *Tai = (zdouble *)malloc(sizeof(zdouble) * Nv * No);
+  double *pt = per_tuple, *ptc = per_tuple_ct;
+  if (!pt) pt = (double *)malloc(sizeof(double) * (count ? count : 1));
+  if (!ptc) ptc = (double *)malloc(sizeof(double) * (count ? count : 1));
+  if (!Tai || !pt || !ptc) return -1;
+  for (size_t q = 0; q < Nv * No; q++)
+    Tai[q] = ijkabc ? -((const zdouble *)in->Tai)[q] : ((const zdouble *)in->Tai)[q];
+  worker_z_args *args = (worker_z_args *)calloc(nthreads, sizeof(worker_z_args));
+  pthread_t *th = (pthread_t *)calloc(nthreads, sizeof(pthread_t));
+  const uint64_t chunk = (count + nthreads - 1) / nthreads;
+  int status = 0;
+  for (int t = 0; t < nthreads; t++) {
+    worker_z_args *w = &args[t];
+    w->in = in;
+    w->Tai = Tai;
+    w->first = first;
+    w->lo = first + chunk * t;
+    w->hi = w->lo + chunk;
+    if (w->lo > first + count) w->lo = first + count;
+    if (w->hi > first + count) w->hi = first + count;
+    w->pt = pt;
+    w->ptc = ptc;
+    if (nthreads == 1)
+      worker_z(w);
+    else
+      pthread_create(&th[t], NULL, worker_z, w);
+  }
+  if (nthreads > 1)
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  for (int t = 0; t < nthreads; t++) status |= args[t].status;
+  double e = 0.0, ect = 0.0;
+  for (uint64_t q = 0; q < count; q++) {
+    e += pt[q];
+    ect += ptc[q];
+  }
+  if (!ijkabc) {
+    e = -e;
+    ect = -ect;
+  }
+  *energy = e;
+  *ct_energy = ect;
+  if (pt != per_tuple) free(pt);
+  if (ptc != per_tuple_ct) free(ptc);
+  free(args);
+  free(th);
   free(Tai);
   return status;
 }

@@ -130,6 +130,28 @@ int oracle_run(const oracle_input *in, int ijkabc, uint64_t first,
                double *energy, double *ct_energy, double *per_tuple,
                double *per_tuple_ct);
 
+/*
+ * Complex (T) (atrip::Atrip::run<Complex>, src/atrip/Atrip.cxx:1136), loop
+ * form.  Tensors are interleaved complex (re, im) in CTF layout, i.e. the
+ * memory of std::complex<double> arrays.  Follows the reference's GEMM
+ * path: the hole terms use the conjugated Vhhh slices (MAYBE_CONJ before
+ * each hole DGEMM pair, src/atrip/Equations.cxx:620-656), the particle terms
+ * and the singles use plain products (zgemm "T"/"N", :484-489, and
+ * :387-426), and the energy conjugates W (maybe_conjugate_scalar on Tijk,
+ * :135-146, :212-217), divides by the complex denominator and keeps the
+ * real part of the tuple's sum (:176-178).  Same selector, tuple order and
+ * sign as oracle_run; per_tuple values are the real parts.
+ */
+typedef struct {
+  size_t No, Nv;
+  const double *ei, *ea, *Tai, *Tabij, *Vabij, *Vijka, *Vabci;
+  const double *Jijka, *Jabci;
+} oracle_input_z;
+
+int oracle_run_z(const oracle_input_z *in, int ijkabc, uint64_t first,
+                 uint64_t count, int nthreads, double *energy,
+                 double *ct_energy, double *per_tuple, double *per_tuple_ct);
+
 /* Fill a whole tensor in CTF layout from the counter-based generator
  * (include/atrip_synth.h).  n = number of elements of that tensor; the
  * elements are split over nthreads threads. */

@@ -69,6 +69,10 @@ def lib():
         L.oracle_energy_distinct.argtypes = [C.c_double, C.c_size_t, dptr, dptr, dptr]
         L.oracle_energy_same.restype = C.c_double
         L.oracle_energy_same.argtypes = [C.c_double, C.c_size_t, dptr, dptr, dptr]
+        L.oracle_run_z.restype = C.c_int
+        L.oracle_run_z.argtypes = [
+            C.POINTER(OracleInput), C.c_int, C.c_uint64, C.c_uint64, C.c_int,
+            dptr, dptr, dptr, dptr]
         _lib = L
     return _lib
 
@@ -114,6 +118,51 @@ def synthetic(no: int, nv: int, seed: int, with_j: bool = False, nthreads: int =
         lib().oracle_fill_synthetic(seed, TENSOR_ID[name], no, nv, _p(arr), n, nthreads)
         out[name] = arr
     return out
+
+
+IMAG = 16  # imaginary-part flag of the generator's tensor ids
+
+
+def synthetic_z(no: int, nv: int, seed: int, with_j: bool = False,
+                nthreads: int = 1) -> dict:
+    """Seeded complex inputs: real parts as synthetic(), imaginary parts from
+    the tensor ids | IMAG (orbital energies real); complex128, CTF layout."""
+    out = {}
+    names = TENSORS if with_j else TENSORS[:7]
+    for name in names:
+        n = tensor_size(name, no, nv)
+        re = np.empty(n, dtype=np.float64)
+        im = np.empty(n, dtype=np.float64)
+        lib().oracle_fill_synthetic(seed, TENSOR_ID[name], no, nv, _p(re), n, nthreads)
+        lib().oracle_fill_synthetic(seed, TENSOR_ID[name] | IMAG, no, nv, _p(im), n,
+                                    nthreads)
+        out[name] = re + 1j * im
+    return out
+
+
+def run_z(no: int, nv: int, t: dict, *, ijkabc=False, first=0, count=None,
+          nthreads=1, per_tuple=False):
+    """Complex (T) loop (oracle_run_z); complex128 inputs in CTF layout."""
+    n = n_tuples(nv)
+    if count is None:
+        count = n - first
+    keep = {k: np.ascontiguousarray(v, dtype=np.complex128) for k, v in t.items()}
+    inp = OracleInput()
+    inp.No, inp.Nv = no, nv
+    for name in TENSORS:
+        arr = keep.get(name)
+        setattr(inp, name, arr.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
+                if arr is not None else C.POINTER(C.c_double)())
+    e, ect = C.c_double(), C.c_double()
+    pt = np.zeros(max(count, 1))
+    ptc = np.zeros(max(count, 1))
+    rc = lib().oracle_run_z(C.byref(inp), int(ijkabc), first, count, nthreads,
+                            C.byref(e), C.byref(ect), _p(pt), _p(ptc))
+    if rc != 0:
+        raise RuntimeError(f"oracle_run_z failed ({rc})")
+    if per_tuple:
+        return e.value, ect.value, pt[:count], ptc[:count]
+    return e.value, ect.value
 
 
 def make_input(no: int, nv: int, t: dict) -> OracleInput:

@@ -195,3 +195,89 @@ def test_cT_with_J_equal_V(oracle):
     # linear in J: E_ct(2 J) = 2 E_ct(J) - ... bilinear in (W_J, Z): scales by 2
     dbl = dict(t, Jabci=2 * t["Jabci"], Jijka=2 * t["Jijka"])
     assert abs(oracle.run(no, nv, dbl)[1] - 2 * ect) <= 1e-12 * abs(ect)
+
+
+# ---- complex (T) ---------------------------------------------------------
+
+def _full_space_energy_z(no, nv, t, conj_holes=True, conj_w=True):
+    """Complex analogue of _full_space_energy: hole terms with conj(Vhhh),
+    energy -(1/6) Re sum 2 conj(W) Y / D.  The switches drop either
+    conjugation (used to show the identity tells them apart)."""
+    T = t["Tabij"].reshape((nv, nv, no, no), order="F")
+    Vab = t["Vabij"].reshape((nv, nv, no, no), order="F")
+    Vh = t["Vijka"].reshape((no, no, no, nv), order="F")
+    Vh = np.conj(Vh) if conj_holes else Vh
+    Vp = t["Vabci"].reshape((nv, nv, nv, no), order="F")
+    tai = t["Tai"].reshape((nv, no), order="F")
+    ei, ea = t["ei"], t["ea"]
+    es = np.einsum
+    W = (es("aEij,bcEk->abcijk", T, Vp) + es("aEik,cbEj->abcijk", T, Vp)
+         + es("cEki,abEj->abcijk", T, Vp) + es("cEkj,baEi->abcijk", T, Vp)
+         + es("bEji,acEk->abcijk", T, Vp) + es("bEjk,caEi->abcijk", T, Vp)
+         - es("abLj,ikLc->abcijk", T, Vh) - es("abiL,jkLc->abcijk", T, Vh)
+         - es("acLk,ijLb->abcijk", T, Vh) - es("aciL,kjLb->abcijk", T, Vh)
+         - es("bcLk,jiLa->abcijk", T, Vh) - es("bcjL,kiLa->abcijk", T, Vh))
+    S = (es("ai,bcjk->abcijk", tai, Vab) + es("bj,acik->abcijk", tai, Vab)
+         + es("ck,abij->abcijk", tai, Vab))
+    Z = W + S
+    tr = lambda X, p: X.transpose(0, 1, 2, *[3 + q for q in p])
+    Y = (4 * Z + tr(Z, (1, 2, 0)) + tr(Z, (2, 0, 1)) - 2 * tr(Z, (0, 2, 1))
+         - 2 * tr(Z, (1, 0, 2)) - 2 * tr(Z, (2, 1, 0)))
+    D = (ea[:, None, None, None, None, None] + ea[None, :, None, None, None, None]
+         + ea[None, None, :, None, None, None] - ei[None, None, None, :, None, None]
+         - ei[None, None, None, None, :, None] - ei[None, None, None, None, None, :])
+    Wc = np.conj(W) if conj_w else W
+    return -np.real(np.sum(2 * Wc * Y / D)) / 6.0
+
+
+@pytest.mark.parametrize("no,nv", SIZES)
+def test_complex_full_space_identity(oracle, no, nv):
+    t = oracle.synthetic_z(no, nv, seed=3)
+    e, ect = oracle.run_z(no, nv, t)
+    assert ect == e
+    e_full = _full_space_energy_z(no, nv, t)
+    assert abs(e - e_full) <= 1e-12 * abs(e_full) + 1e-18
+
+
+def test_complex_identity_sees_each_conjugation(oracle):
+    no, nv = 4, 7
+    t = oracle.synthetic_z(no, nv, seed=3)
+    e, _ = oracle.run_z(no, nv, t)
+    for kw in ({"conj_holes": False}, {"conj_w": False}):
+        assert abs(e - _full_space_energy_z(no, nv, t, **kw)) > 1e-5 * abs(e), kw
+
+
+def test_complex_with_real_inputs_is_the_real_path(oracle):
+    for no, nv in [(3, 5), (5, 9)]:
+        t = oracle.synthetic(no, nv, seed=8, with_j=True)
+        tz = {k: v.astype(np.complex128) for k, v in t.items()}
+        e, ect, p, pc = oracle.run(no, nv, t, per_tuple=True)
+        ez, ectz, pz, pcz = oracle.run_z(no, nv, tz, per_tuple=True)
+        assert np.max(np.abs(pz - p)) <= 1e-13 * np.max(np.abs(p))
+        assert abs(ez - e) <= 1e-13 * abs(e) and abs(ectz - ect) <= 1e-13 * abs(ect)
+
+
+def test_complex_generator_parts(oracle):
+    t = oracle.synthetic(5, 8, seed=4, with_j=True)
+    tz = oracle.synthetic_z(5, 8, seed=4, with_j=True)
+    for k in t:
+        assert np.array_equal(tz[k].real, t[k])
+    assert not np.any(tz["ei"].imag) and not np.any(tz["ea"].imag)
+    for k in ("Tai", "Tabij", "Vabij", "Vijka", "Vabci", "Jijka", "Jabci"):
+        assert np.std(tz[k].imag) > 0.5 * np.std(tz[k].real)
+    T = tz["Tabij"].reshape((8, 8, 5, 5), order="F")
+    assert np.array_equal(T, T.transpose(1, 0, 3, 2))  # Tabij = Tbaji
+
+
+def test_complex_cT_and_ijkabc(oracle):
+    no, nv = 4, 6
+    t = oracle.synthetic_z(no, nv, seed=9, with_j=True)
+    tJ = dict(t)
+    tJ["Jijka"], tJ["Jabci"] = t["Vijka"], t["Vabci"]
+    e, ect = oracle.run_z(no, nv, tJ)
+    assert abs(ect - e) <= 1e-14 * abs(e)  # J = V gives E(cT) = E(T)
+    e1, _ = oracle.run_z(no, nv, t, ijkabc=True)
+    tn = dict(t)
+    tn["Tai"] = -t["Tai"]
+    e2, _ = oracle.run_z(no, nv, tn)
+    assert abs(e1 + e2) <= 1e-14 * abs(e1)  # ijkabc: Tai negated, no final sign
