@@ -12,6 +12,7 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/atrip_hip.h"
@@ -48,23 +49,59 @@ struct Ctx {
 };
 
 template <typename F>
-std::vector<double> read_tensor(CTF::Tensor<F> *t, const char *name,
-                                size_t expect) {
+std::vector<F> read_tensor(CTF::Tensor<F> *t, const char *name,
+                           size_t expect) {
   if (!t) throw std::string("atrip: missing input tensor ") + name;
   size_t n = 1;
   for (int d = 0; d < t->order; d++) n *= (size_t)t->lens[d];
   if (n != expect)
     throw std::string("atrip: tensor ") + name + " has " + std::to_string(n) +
         " elements, expected " + std::to_string(expect);
-  std::vector<double> v(n);
+  std::vector<F> v(n);
   t->read_all(v.data());
   return v;
 }
+
+// Host memory of a tensor as the C-ABI sees it: doubles, interleaved
+// (re, im) for complex tensors (std::complex<double> is two doubles).
+template <typename F>
+const double *raw(const std::vector<F> &v) {
+  return reinterpret_cast<const double *>(v.data());
+}
+
+template <typename F>
+int upload(atrip_hip_ctx *h, const atrip_sys_desc *d, int *sys);
+template <>
+int upload<double>(atrip_hip_ctx *h, const atrip_sys_desc *d, int *sys) {
+  return atrip_hip_upload(h, d, sys);
+}
+template <>
+int upload<Complex>(atrip_hip_ctx *h, const atrip_sys_desc *d, int *sys) {
+  return atrip_hip_upload_complex(h, d, sys);
+}
+
+template <typename F>
+Atrip::Output run_impl(Atrip::Input<F> const &in, size_t np);
 
 }  // namespace
 
 template <>
 Atrip::Output Atrip::run<double>(Atrip::Input<double> const &in) {
+  return run_impl<double>(in, np);
+}
+
+// Reference: explicit instantiation for Complex (src/atrip/Atrip.cxx:1136).
+template <>
+Atrip::Output Atrip::run<Complex>(Atrip::Input<Complex> const &in) {
+  if (in.mixed_precision)
+    throw std::string("atrip: the complex (T) path runs in FP64 only");
+  return run_impl<Complex>(in, np);
+}
+
+namespace {
+
+template <typename F>
+Atrip::Output run_impl(Atrip::Input<F> const &in, size_t np) {
   if (!in.ei || !in.ea) throw std::string("atrip: epsilon tensors missing");
   const size_t No = (size_t)in.ei->lens[0];
   const size_t Nv = (size_t)in.ea->lens[0];
@@ -73,14 +110,14 @@ Atrip::Output Atrip::run<double>(Atrip::Input<double> const &in) {
             << "Atrip: np: " << np << "\n";
 
   // host copies in CTF layout (the reference's read_all, :179-181)
-  std::vector<double> ei = read_tensor(in.ei, "ei", No);
-  std::vector<double> ea = read_tensor(in.ea, "ea", Nv);
-  std::vector<double> tai = read_tensor(in.Tph, "Tai", Nv * No);
-  std::vector<double> tabij = read_tensor(in.Tpphh, "Tabij", Nv * Nv * No * No);
-  std::vector<double> vabij = read_tensor(in.Vpphh, "Vabij", Nv * Nv * No * No);
-  std::vector<double> vijka = read_tensor(in.Vhhhp, "Vijka", No * No * No * Nv);
-  std::vector<double> vabci = read_tensor(in.Vppph, "Vabci", Nv * Nv * Nv * No);
-  std::vector<double> jijka, jabci;
+  std::vector<F> ei = read_tensor(in.ei, "ei", No);
+  std::vector<F> ea = read_tensor(in.ea, "ea", Nv);
+  std::vector<F> tai = read_tensor(in.Tph, "Tai", Nv * No);
+  std::vector<F> tabij = read_tensor(in.Tpphh, "Tabij", Nv * Nv * No * No);
+  std::vector<F> vabij = read_tensor(in.Vpphh, "Vabij", Nv * Nv * No * No);
+  std::vector<F> vijka = read_tensor(in.Vhhhp, "Vijka", No * No * No * Nv);
+  std::vector<F> vabci = read_tensor(in.Vppph, "Vabci", Nv * Nv * Nv * No);
+  std::vector<F> jijka, jabci;
   const bool with_j = in.Jhhhp && in.Jppph;
   if (with_j) {
     jijka = read_tensor(in.Jhhhp, "Jijka", No * No * No * Nv);
@@ -91,20 +128,20 @@ Atrip::Output Atrip::run<double>(Atrip::Input<double> const &in) {
   atrip_sys_desc d{};
   d.No = (int64_t)No;
   d.Nv = (int64_t)Nv;
-  d.ei = ei.data();
-  d.ea = ea.data();
-  d.Tai = tai.data();
-  d.Tabij = tabij.data();
-  d.Vabij = vabij.data();
-  d.Vijka = vijka.data();
-  d.Vabci = vabci.data();
-  d.Jijka = with_j ? jijka.data() : nullptr;
-  d.Jabci = with_j ? jabci.data() : nullptr;
+  d.ei = raw(ei);
+  d.ea = raw(ea);
+  d.Tai = raw(tai);
+  d.Tabij = raw(tabij);
+  d.Vabij = raw(vabij);
+  d.Vijka = raw(vijka);
+  d.Vabci = raw(vabci);
+  d.Jijka = with_j ? raw(jijka) : nullptr;
+  d.Jabci = with_j ? raw(jabci) : nullptr;
   int sys = 0;
-  ctx.check(atrip_hip_upload(ctx.h, &d, &sys), "upload");
+  ctx.check(upload<F>(ctx.h, &d, &sys), "upload");
   // host copies are not needed once resident
-  std::vector<double>().swap(vabci);
-  std::vector<double>().swap(jabci);
+  std::vector<F>().swap(vabci);
+  std::vector<F>().swap(jabci);
   if (in.delete_Vppph) delete in.Vppph;  // reference: src/atrip/Atrip.cxx:310
 
   const uint64_t n = (uint64_t)Nv * (Nv + 1) * (Nv + 2) / 6 - Nv;
@@ -180,7 +217,7 @@ Atrip::Output Atrip::run<double>(Atrip::Input<double> const &in) {
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
           .count();
 
-  Output out{energy_base + raw, raw_ct};
+  Atrip::Output out{energy_base + raw, raw_ct};
   if (!in.ijkabc) {
     out.energy = -out.energy;
     out.ct_energy = -out.ct_energy;
@@ -189,20 +226,15 @@ Atrip::Output Atrip::run<double>(Atrip::Input<double> const &in) {
             << out.energy << std::endl;
   std::cout << "Atrip: Energy (cT): " << std::setprecision(15) << std::setw(23)
             << out.ct_energy << std::endl;
-  const double gflop_tuple =
-      12.0 * (double)No * No * No * (double)(No + Nv) / 1e9;
+  // the reference counts complex products as four real ones (bench/main.cxx:119)
+  const double gflop_tuple = 12.0 * (double)No * No * No * (double)(No + Nv) /
+                             1e9 * (std::is_same<F, Complex>::value ? 4.0 : 1.0);
   if (elapsed > 0)
     std::cout << "atrip:flops(iterations): "
               << gflop_tuple * (double)(last - first) / elapsed << "\n";
   return out;
 }
 
-// Reference: explicit instantiation for Complex (src/atrip/Atrip.cxx:1136).
-template <>
-Atrip::Output Atrip::run<Complex>(Atrip::Input<Complex> const &) {
-  throw std::string(
-      "atrip: the complex (T) path is not available in this build; "
-      "run<double> only");
-}
+}  // namespace
 
 }  // namespace atrip

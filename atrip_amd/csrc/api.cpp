@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "host.hpp"
+#include "../../include/atrip_synth.h"
 
 using namespace atrip_amd;
 
@@ -95,8 +96,10 @@ uint64_t atrip_amd::system_bytes(int64_t No, int64_t Nv, bool with_j,
 namespace {
 
 void alloc_system(atrip_hip_ctx *ctx, DevSystem &s, int64_t No, int64_t Nv,
-                  bool with_j) {
+                  bool with_j, bool cplx = false) {
   if (No < 1 || Nv < 1) throw Fail{"No and Nv must be positive"};
+  if (cplx && ctx->operand_store != 0)
+    throw Fail{"complex systems keep FP64 operands (operand_store 0)"};
   if (No > 1023) throw Fail{"No > 1023 is not supported"};
   s.No = No;
   s.Nv = Nv;
@@ -131,6 +134,25 @@ void alloc_system(atrip_hip_ctx *ctx, DevSystem &s, int64_t No, int64_t Nv,
   if (with_j) {
     operand(s.V3J, s.sV3J, Nv2 * No * s.Kv, "alloc V3J");
     operand(s.H3J, s.sH3J, Nv * N2 * s.Ko, "alloc H3J");
+  }
+  s.cplx = cplx;
+  if (cplx) {
+    s.T3i = dev_alloc(s, Nv * N2 * s.Kv, "alloc T3 (im)");
+    s.V3i = dev_alloc(s, Nv2 * No * s.Kv, "alloc V3 (im)");
+    s.H3i = dev_alloc(s, Nv * N2 * s.Ko, "alloc H3 (im)");
+    s.T2Ni = dev_alloc(s, Nv2 * No * s.Ko, "alloc T2N (im)");
+    s.T2TNi = dev_alloc(s, Nv2 * No * s.Ko, "alloc T2TN (im)");
+    s.VHHi = dev_alloc(s, Nv2 * N2, "alloc VHH (im)");
+    s.Taii = dev_alloc(s, Nv * No, "alloc Tai (im)");
+    s.eii = dev_alloc(s, No, "alloc ei (im)");
+    s.eai = dev_alloc(s, Nv, "alloc ea (im)");
+    s.T3in = dev_alloc(s, Nv * N2 * s.Kv, "alloc -T3 (im)");
+    s.H3in = dev_alloc(s, Nv * N2 * s.Ko, "alloc -H3 (im)");
+    if (with_j) {
+      s.V3Ji = dev_alloc(s, Nv2 * No * s.Kv, "alloc V3J (im)");
+      s.H3Ji = dev_alloc(s, Nv * N2 * s.Ko, "alloc H3J (im)");
+      s.H3Jin = dev_alloc(s, Nv * N2 * s.Ko, "alloc -H3J (im)");
+    }
   }
   // i >= j >= k points of the energy loops, packed 10 bits per index
   std::vector<uint32_t> pts;
@@ -309,6 +331,54 @@ void upload_host(atrip_hip_ctx *ctx, DevSystem &s, const atrip_sys_desc *d) {
         "upload ea");
 }
 
+// The imaginary planes of a complex system in the slots of the real ones
+// (for filling them with the real-system code).
+DevSystem imag_view(const DevSystem &s) {
+  DevSystem v;
+  v.No = s.No; v.Nv = s.Nv; v.Kv = s.Kv; v.Ko = s.Ko;
+  v.has_j = s.has_j;
+  v.T3 = s.T3i; v.V3 = s.V3i; v.H3 = s.H3i; v.T2N = s.T2Ni;
+  v.T2TN = s.T2TNi; v.VHH = s.VHHi; v.Tai = s.Taii; v.ei = s.eii;
+  v.ea = s.eai; v.V3J = s.V3Ji; v.H3J = s.H3Ji;
+  v.aprefix = s.aprefix;
+  return v;
+}
+
+// Negated imaginary planes of T3 and the hole integrals.
+void complex_negations(atrip_hip_ctx *ctx, DevSystem &s) {
+  const int64_t N2 = s.No * s.No;
+  check(launch_scale(s.T3i, s.T3in, s.Nv * N2 * s.Kv, -1.0, ctx->stream),
+        "negate T3 (im)");
+  check(launch_scale(s.H3i, s.H3in, s.Nv * N2 * s.Ko, -1.0, ctx->stream),
+        "negate H3 (im)");
+  if (s.has_j)
+    check(launch_scale(s.H3Ji, s.H3Jin, s.Nv * N2 * s.Ko, -1.0, ctx->stream),
+          "negate H3J (im)");
+  check(hipStreamSynchronize(ctx->stream), "negations");
+}
+
+// The four real class-GEMM passes of a complex system: W = Wr + i Wi with
+//   Wr = [T3r V3r + H3r N_r] + [(-T3i) V3i + H3i N_i]
+//   Wi = [T3r V3i + H3r N_i] + [T3i V3r + (-H3i) N_r]
+// where N = -T2 (the pre-negated hole operands) and the hole terms carry
+// conj(Vhhh) = H3r - i H3i, as the reference's complex GEMM path does
+// (MAYBE_CONJ, src/atrip/Equations.cxx:620-656).
+void complex_passes(const DevSystem &s, DevSystem v[4]) {
+  for (int q = 0; q < 4; q++) {
+    v[q].No = s.No; v[q].Nv = s.Nv; v[q].Kv = s.Kv; v[q].Ko = s.Ko;
+    v[q].aprefix = s.aprefix;
+    v[q].has_j = s.has_j;
+  }
+  v[0].T3 = s.T3;   v[0].V3 = s.V3;  v[0].H3 = s.H3;   v[0].T2N = s.T2N;
+  v[0].T2TN = s.T2TN;  v[0].V3J = s.V3J;  v[0].H3J = s.H3J;
+  v[1].T3 = s.T3in; v[1].V3 = s.V3i; v[1].H3 = s.H3i;  v[1].T2N = s.T2Ni;
+  v[1].T2TN = s.T2TNi; v[1].V3J = s.V3Ji; v[1].H3J = s.H3Ji;
+  v[2].T3 = s.T3;   v[2].V3 = s.V3i; v[2].H3 = s.H3;   v[2].T2N = s.T2Ni;
+  v[2].T2TN = s.T2TNi; v[2].V3J = s.V3Ji; v[2].H3J = s.H3J;
+  v[3].T3 = s.T3i;  v[3].V3 = s.V3;  v[3].H3 = s.H3in; v[3].T2N = s.T2N;
+  v[3].T2TN = s.T2TN;  v[3].V3J = s.V3J;  v[3].H3J = s.H3Jin;
+}
+
 // Host-side canonical index, identical to the device one.
 struct HostIndex {
   std::vector<uint64_t> pre;
@@ -387,6 +457,8 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
       precision != ATRIP_FP32)
     throw Fail{"unknown precision " + std::to_string(precision)};
   const bool g32 = precision != ATRIP_FP64;
+  if (s.cplx && precision != ATRIP_FP64)
+    throw Fail{"complex systems run in FP64 only"};
   if (precision == ATRIP_FP64 && s.store != 0)
     throw Fail{"system was uploaded without FP64 operands (operand_store)"};
   if (precision == ATRIP_SPLIT_FP32 && s.store == 2)
@@ -442,7 +514,17 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   const bool use_j = s.has_j;
   const size_t gsz =
       (size_t)(3 * slots_max * W3) * (g32 ? sizeof(float) : sizeof(double));
-  ensure((void *&)ctx->G, ctx->G_bytes, use_j ? 2 * gsz : gsz, "alloc G");
+  // complex: four class-buffer sets per pass (Wr = G0 + G1, Wi = G2 + G3)
+  const int nsets = s.cplx ? 4 : 1;
+  ensure((void *&)ctx->G, ctx->G_bytes, (use_j ? 2 : 1) * nsets * gsz,
+         "alloc G");
+  DevSystem zpass[4];
+  if (s.cplx) complex_passes(s, zpass);
+  double *Gz[4], *GJz[4];
+  for (int q = 0; q < 4; q++) {
+    Gz[q] = (double *)((char *)ctx->G + q * gsz);
+    GJz[q] = (double *)((char *)ctx->G + (4 + q) * gsz);
+  }
   ensure((void *&)ctx->e_slot, ctx->e_slot_bytes,
          (size_t)(2 * slots_max) * sizeof(double), "alloc e_slot");
   const int nbatch = (nsel + bw - 1) / bw;
@@ -450,7 +532,9 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
          (size_t)(4 * nbatch) * sizeof(double), "alloc partials");
   void *G = ctx->G;
   void *GJ = use_j ? (void *)((char *)ctx->G + gsz) : nullptr;
-  const double flop_tuple = 12.0 * (double)s.No * s.No * s.No * (s.No + s.Nv);
+  // complex products are four real ones (the reference bench counts 4x)
+  const double flop_tuple = 12.0 * (double)s.No * s.No * s.No * (s.No + s.Nv) *
+                            (s.cplx ? 4.0 : 1.0);
   const char *dname = precision == ATRIP_FP64        ? "doubles"
                       : precision == ATRIP_SPLIT_FP32 ? "doubles_mixed"
                                                       : "doubles_fp32";
@@ -474,7 +558,13 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
       check(hipEventCreate(&e2), "event");
       check(hipEventRecord(e0, ctx->stream), "event");
     }
-    for (int pass = 0; pass < (use_j ? 2 : 1); pass++) {
+    for (int pass = 0; pass < (use_j ? 2 : 1) && s.cplx; pass++)
+      for (int q = 0; q < 4; q++)
+        check(launch_doubles_f64(zpass[q], b, pass == 1,
+                                 pass == 0 ? Gz[q] : GJz[q], ctx->tile,
+                                 ctx->stream),
+              "doubles kernel (complex pass)");
+    for (int pass = 0; pass < (use_j ? 2 : 1) && !s.cplx; pass++) {
       void *Gp = pass == 0 ? G : GJ;
       if (precision == ATRIP_FP64)
         check(launch_doubles_f64(s, b, pass == 1, (double *)Gp, ctx->tile,
@@ -487,9 +577,14 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
               "doubles (FP32 products) kernel");
     }
     if (ctx->timing) check(hipEventRecord(e1, ctx->stream), "event");
-    check(launch_energy(s, b, G, GJ, g32, ijkabc ? -1.0 : 1.0, ctx->e_slot,
-                        ctx->stream),
-          "energy kernel");
+    if (s.cplx)
+      check(launch_energy_z(s, b, Gz, use_j ? GJz : nullptr,
+                            ijkabc ? -1.0 : 1.0, ctx->e_slot, ctx->stream),
+            "energy kernel (complex)");
+    else
+      check(launch_energy(s, b, G, GJ, g32, ijkabc ? -1.0 : 1.0, ctx->e_slot,
+                          ctx->stream),
+            "energy kernel");
     if (ctx->timing) {
       check(hipEventRecord(e2, ctx->stream), "event");
       ctx->pending_events.push_back({e0, e1});
@@ -649,6 +744,83 @@ int atrip_hip_upload_synthetic(atrip_hip_ctx *ctx, int64_t No, int64_t Nv,
       alloc_system(ctx, s, No, Nv, with_j != 0);
       check(launch_synth(s, seed, with_j != 0, ctx->stream), "synthetic fill");
       check(hipStreamSynchronize(ctx->stream), "synthetic fill");
+    } catch (...) {
+      free_system(s);
+      throw;
+    }
+    const int id = ctx->next_sys++;
+    ctx->systems.emplace(id, std::move(s));
+    *sys = id;
+  });
+}
+
+int atrip_hip_upload_complex(atrip_hip_ctx *ctx, const atrip_sys_desc *desc,
+                             int *sys) {
+  return guarded(ctx, [&] {
+    if (!desc || !sys) throw Fail{"null argument"};
+    if (!desc->ei || !desc->ea || !desc->Tai || !desc->Tabij ||
+        !desc->Vabij || !desc->Vijka || !desc->Vabci)
+      throw Fail{"missing input tensor"};
+    if ((desc->Jijka == nullptr) != (desc->Jabci == nullptr))
+      throw Fail{"Jijka and Jabci must be given together"};
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    const int64_t No = desc->No, Nv = desc->Nv;
+    const bool with_j = desc->Jijka != nullptr;
+    DevSystem s;
+    try {
+      alloc_system(ctx, s, No, Nv, with_j, true);
+      // real, then imaginary parts through the real upload path
+      const int64_t N2 = No * No, Nv2 = Nv * Nv;
+      const double *src[9] = {desc->ei,    desc->ea,    desc->Tai,
+                              desc->Tabij, desc->Vabij, desc->Vijka,
+                              desc->Vabci, desc->Jijka, desc->Jabci};
+      const int64_t len[9] = {No,         Nv,        Nv * No,
+                              Nv2 * N2,   Nv2 * N2,  N2 * No * Nv,
+                              Nv2 * Nv * No, N2 * No * Nv, Nv2 * Nv * No};
+      for (int part = 0; part < 2; part++) {
+        std::vector<std::vector<double>> host(9);
+        for (int t = 0; t < 9; t++) {
+          if (!src[t]) continue;
+          host[t].resize(len[t]);
+          for (int64_t q = 0; q < len[t]; q++) host[t][q] = src[t][2 * q + part];
+        }
+        atrip_sys_desc d{No, Nv, host[0].data(), host[1].data(), host[2].data(),
+                         host[3].data(), host[4].data(), host[5].data(),
+                         host[6].data(), with_j ? host[7].data() : nullptr,
+                         with_j ? host[8].data() : nullptr};
+        if (part == 0) {
+          upload_host(ctx, s, &d);
+        } else {
+          DevSystem v = imag_view(s);
+          upload_host(ctx, v, &d);
+        }
+      }
+      complex_negations(ctx, s);
+    } catch (...) {
+      free_system(s);
+      throw;
+    }
+    const int id = ctx->next_sys++;
+    ctx->systems.emplace(id, std::move(s));
+    *sys = id;
+  });
+}
+
+int atrip_hip_upload_synthetic_complex(atrip_hip_ctx *ctx, int64_t No,
+                                       int64_t Nv, uint64_t seed, int with_j,
+                                       int *sys) {
+  return guarded(ctx, [&] {
+    if (!sys) throw Fail{"null argument"};
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    DevSystem s;
+    try {
+      alloc_system(ctx, s, No, Nv, with_j != 0, true);
+      check(launch_synth(s, seed, with_j != 0, ctx->stream), "synthetic fill");
+      DevSystem v = imag_view(s);
+      check(launch_synth(v, seed, with_j != 0, ctx->stream, ATRIP_SYN_IMAG),
+            "synthetic fill (im)");
+      check(hipStreamSynchronize(ctx->stream), "synthetic fill");
+      complex_negations(ctx, s);
     } catch (...) {
       free_system(s);
       throw;

@@ -51,6 +51,14 @@ struct DevSystem {
          *ea = nullptr;
   double *V3J = nullptr, *H3J = nullptr;  // (cT) integrals, may be null
   bool has_j = false;                     // (cT) integrals present
+  // Complex systems: the arrays above hold the real parts, these the
+  // imaginary parts (same layouts), and the *n arrays the negated imaginary
+  // parts of T3 and the hole integrals (operands of the real-GEMM passes).
+  bool cplx = false;
+  double *T3i = nullptr, *V3i = nullptr, *H3i = nullptr, *T2Ni = nullptr,
+         *T2TNi = nullptr, *VHHi = nullptr, *Taii = nullptr, *eii = nullptr,
+         *eai = nullptr, *V3Ji = nullptr, *H3Ji = nullptr;
+  double *T3in = nullptr, *H3in = nullptr, *H3Jin = nullptr;
   // Split-FP32 copies of the GEMM operands, made on the first mixed run:
   // h = (float)x, b = bf16(h), l = bf16(x - h), same layouts as above.
   struct Split {
@@ -182,6 +190,12 @@ hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
                          const void *G, const void *GJ, bool g32,
                          double tai_sign, double *e_slot, hipStream_t st);
 
+// Singles + energy of complex systems from the four class-buffer sets of
+// the real passes (Wr = G0 + G1, Wi = G2 + G3); GJ null without (cT).
+hipError_t launch_energy_z(const DevSystem &s, const BatchArgs &b,
+                           double *const G[4], double *const GJ[4],
+                           double tai_sign, double *e_slot, hipStream_t st);
+
 // Deterministic compensated sum of the valid slot energies of a batch into
 // partials[4*index + {sum, comp, sum_ct, comp_ct}].
 hipError_t launch_batch_reduce(const DevSystem &s, const BatchArgs &b,
@@ -194,8 +208,14 @@ hipError_t launch_transpose(const double *src, double *dst, int64_t n_outer,
                             int64_t src_a, int64_t src_b, int64_t dst_o,
                             int64_t dst_a, int64_t dst_b, double scale,
                             hipStream_t st);
+// Generate a synthetic system in place; tensor_flag is or-ed into every
+// tensor id (ATRIP_SYN_IMAG fills the imaginary parts of a complex system,
+// given a DevSystem whose arrays are its imaginary planes).
 hipError_t launch_synth(DevSystem &s, uint64_t seed, bool with_j,
-                        hipStream_t st);
+                        hipStream_t st, int tensor_flag = 0);
+// dst[q] = scale * src[q], q < n.
+hipError_t launch_scale(const double *src, double *dst, int64_t n,
+                        double scale, hipStream_t st);
 // Fill a Split copy of n doubles.
 hipError_t launch_split(const double *x, DevSystem::Split &out, int64_t n,
                         hipStream_t st);

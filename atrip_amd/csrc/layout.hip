@@ -139,7 +139,7 @@ hipError_t launch_transpose(const double *src, double *dst, int64_t n_outer,
 }
 
 hipError_t launch_synth(DevSystem &s, uint64_t seed, bool with_j,
-                        hipStream_t st) {
+                        hipStream_t st, int tensor_flag) {
   const int64_t No = s.No, Nv = s.Nv, N2 = No * No, Nv2 = Nv * Nv;
   hipError_t e;
   const DevSystem::Split none;
@@ -147,10 +147,11 @@ hipError_t launch_synth(DevSystem &s, uint64_t seed, bool with_j,
   // the system keeps no FP64 copy
 #define SYN(ptr, sp, n, tgt, tensor)                                        \
   if ((e = synth_fill(s.store ? nullptr : ptr, s.store ? sp : none, n, tgt, \
-                      tensor, seed, s, st)) != hipSuccess)                  \
+                      (tensor) | tensor_flag, seed, s, st)) != hipSuccess)  \
     return e;
 #define SYN64(ptr, n, tgt, tensor)                                         \
-  if ((e = synth_fill(ptr, none, n, tgt, tensor, seed, s, st)) != hipSuccess) \
+  if ((e = synth_fill(ptr, none, n, tgt, (tensor) | tensor_flag, seed, s,  \
+                      st)) != hipSuccess)                                   \
     return e;
   SYN(s.T3, s.sT3, Nv * N2 * s.Kv, S_T3, ATRIP_SYN_TABIJ);
   SYN(s.V3, s.sV3, Nv2 * No * s.Kv, S_V3, ATRIP_SYN_VABCI);
@@ -168,6 +169,24 @@ hipError_t launch_synth(DevSystem &s, uint64_t seed, bool with_j,
 #undef SYN
 #undef SYN64
   return hipSuccess;
+}
+
+__global__ __launch_bounds__(256) void scale_kernel(const double *src,
+                                                    double *dst, int64_t n,
+                                                    double scale) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x)
+    dst[q] = scale * src[q];
+}
+
+hipError_t launch_scale(const double *src, double *dst, int64_t n,
+                        double scale, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536L * 16) blocks = 65536L * 16;
+  hipLaunchKernelGGL(scale_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
+                     src, dst, n, scale);
+  return hipGetLastError();
 }
 
 }  // namespace atrip_amd

@@ -24,6 +24,7 @@ PRECISIONS = {"fp64": FP64, "split_fp32": SPLIT_FP32, "mixed": SPLIT_FP32,
 SYMBOLS = (
     "atrip_hip_create", "atrip_hip_destroy", "atrip_hip_last_error",
     "atrip_hip_upload", "atrip_hip_upload_synthetic", "atrip_hip_release",
+    "atrip_hip_upload_complex", "atrip_hip_upload_synthetic_complex",
     "atrip_hip_run", "atrip_hip_tuple_energies", "atrip_hip_set_option",
     "atrip_hip_get_option", "atrip_hip_kernel_stats", "atrip_hip_reset_stats",
     "atrip_hip_sys_info", "atrip_hip_device_count",
@@ -69,6 +70,9 @@ def load():
     L.atrip_hip_upload_synthetic.argtypes = [
         vp, C.c_int64, C.c_int64, C.c_uint64, C.c_int, C.POINTER(C.c_int)]
     L.atrip_hip_release.argtypes = [vp, C.c_int]
+    L.atrip_hip_upload_complex.argtypes = [vp, C.POINTER(SysDesc), C.POINTER(C.c_int)]
+    L.atrip_hip_upload_synthetic_complex.argtypes = [
+        vp, C.c_int64, C.c_int64, C.c_uint64, C.c_int, C.POINTER(C.c_int)]
     L.atrip_hip_run.argtypes = [
         vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
         C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -198,6 +202,30 @@ class Context:
         sys_ = C.c_int()
         self._check(self._L.atrip_hip_upload_synthetic(
             self._h, no, nv, seed, int(with_j), C.byref(sys_)), "upload_synthetic")
+        return sys_.value
+
+    def upload_complex(self, no: int, nv: int, t: dict) -> int:
+        """Upload complex CTF-layout arrays (complex128, first index fastest)."""
+        d = SysDesc()
+        d.No, d.Nv = no, nv
+        keep = []
+        for name in TENSORS:
+            arr = t.get(name)
+            if arr is None:
+                setattr(d, name, C.POINTER(C.c_double)())
+                continue
+            arr = np.ascontiguousarray(arr, dtype=np.complex128).reshape(-1)
+            keep.append(arr)
+            setattr(d, name, arr.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)))
+        sys_ = C.c_int()
+        self._check(self._L.atrip_hip_upload_complex(self._h, C.byref(d), C.byref(sys_)),
+                    "upload_complex")
+        return sys_.value
+
+    def upload_synthetic_complex(self, no: int, nv: int, seed: int, with_j=False) -> int:
+        sys_ = C.c_int()
+        self._check(self._L.atrip_hip_upload_synthetic_complex(
+            self._h, no, nv, seed, int(with_j), C.byref(sys_)), "upload_synthetic_complex")
         return sys_.value
 
     def release(self, sys_: int):

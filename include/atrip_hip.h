@@ -74,6 +74,19 @@ int atrip_hip_upload(atrip_hip_ctx *ctx, const atrip_sys_desc *desc,
 int atrip_hip_upload_synthetic(atrip_hip_ctx *ctx, int64_t No, int64_t Nv,
                                uint64_t seed, int with_j, int *sys);
 
+/* Complex systems (atrip::Atrip::run<Complex>, src/atrip/Atrip.cxx:1136):
+ * the desc pointers address interleaved (re, im) pairs, i.e. the memory of
+ * std::complex<double> CTF tensors.  Runs of a complex system use FP64
+ * (precision ATRIP_FP64); its energies follow the reference's complex GEMM
+ * path (conjugated Vhhh in the hole terms, conj(W) in the energy, real part
+ * of each tuple's sum).  The synthetic variant draws the imaginary parts
+ * from the generator's ATRIP_SYN_IMAG streams (orbital energies real). */
+int atrip_hip_upload_complex(atrip_hip_ctx *ctx, const atrip_sys_desc *desc,
+                             int *sys);
+int atrip_hip_upload_synthetic_complex(atrip_hip_ctx *ctx, int64_t No,
+                                       int64_t Nv, uint64_t seed, int with_j,
+                                       int *sys);
+
 /* Release a resident system's device memory. */
 int atrip_hip_release(atrip_hip_ctx *ctx, int sys);
 

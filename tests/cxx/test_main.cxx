@@ -70,6 +70,54 @@ struct System {
   }
 };
 
+// The same system with complex tensors: real parts as System, imaginary
+// parts from the generator's ATRIP_SYN_IMAG streams.
+struct ComplexSystem {
+  int no, nv;
+  std::vector<CTF::Tensor<atrip::Complex> *> t;
+  std::vector<std::vector<atrip::Complex>> host;
+  ComplexSystem(int no_, int nv_, uint64_t seed, bool with_j) : no(no_), nv(nv_) {
+    const std::vector<std::vector<int>> lens = {
+        {no}, {nv}, {nv, no}, {nv, nv, no, no}, {nv, nv, no, no},
+        {no, no, no, nv}, {nv, nv, nv, no}, {no, no, no, nv}, {nv, nv, nv, no}};
+    for (int k = 0; k < (with_j ? 9 : 7); k++) {
+      auto *x = new CTF::Tensor<atrip::Complex>((int)lens[k].size(), lens[k].data());
+      const size_t n = (size_t)x->size();
+      std::vector<double> re(n), im(n);
+      oracle_fill_synthetic(seed, k, no, nv, re.data(), n, 4);
+      oracle_fill_synthetic(seed, k | ATRIP_SYN_IMAG, no, nv, im.data(), n, 4);
+      std::vector<atrip::Complex> v(n);
+      for (size_t q = 0; q < n; q++) v[q] = atrip::Complex(re[q], im[q]);
+      x->write_all(v.data());
+      t.push_back(x);
+      host.push_back(v);
+    }
+    while (t.size() < 9) t.push_back(nullptr);
+  }
+  atrip::Atrip::Input<atrip::Complex> input() const {
+    return atrip::Atrip::Input<atrip::Complex>()
+        .with_epsilon_i(t[0]).with_epsilon_a(t[1]).with_Tai(t[2])
+        .with_Tabij(t[3]).with_Vabij(t[4]).with_Vijka(t[5]).with_Vabci(t[6])
+        .with_Jijka(t[7]).with_Jabci(t[8])
+        .with_read_checkpoint_if_exists(false).with_writeCheckpoint(false);
+  }
+  std::pair<double, double> oracle() const {
+    oracle_input_z o{};
+    o.No = no;
+    o.Nv = nv;
+    auto p = [&](size_t k) {
+      return k < host.size() ? reinterpret_cast<const double *>(host[k].data())
+                             : nullptr;
+    };
+    o.ei = p(0); o.ea = p(1); o.Tai = p(2); o.Tabij = p(3); o.Vabij = p(4);
+    o.Vijka = p(5); o.Vabci = p(6); o.Jijka = p(7); o.Jabci = p(8);
+    double e = 0, ect = 0;
+    assert(oracle_run_z(&o, 0, 0, oracle_n_tuples(nv), 4, &e, &ect, nullptr,
+                        nullptr) == 0);
+    return {e, ect};
+  }
+};
+
 bool close(double a, double b, double rel = 1e-10) {
   return std::fabs(a - b) <= rel * std::fabs(b);
 }
@@ -182,6 +230,13 @@ int main(int argc, char **argv) {
            assert(close(out.energy, ref.first));
            assert(close(out.ct_energy, ref.second));)
 
+  TESTCASE("run<Complex> matches the complex oracle, with (cT)",
+           ComplexSystem s(5, 9, 27, true);
+           auto ref = s.oracle();
+           auto out = atrip::Atrip::run(s.input());
+           assert(close(out.energy, ref.first));
+           assert(close(out.ct_energy, ref.second));)
+
   TESTCASE("errors are std::string",
            bool thrown = false;
            try {
@@ -195,6 +250,14 @@ int main(int argc, char **argv) {
            try {
              atrip::Atrip::Input<atrip::Complex> in;
              atrip::Atrip::run(in);
+           } catch (const std::string &e) {
+             thrown = true;
+           }
+           assert(thrown);
+           thrown = false;
+           try {
+             ComplexSystem s(3, 5, 28, false);
+             atrip::Atrip::run(s.input().with_mixed_precision(true));
            } catch (const std::string &e) {
              thrown = true;
            }

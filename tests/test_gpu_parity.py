@@ -239,3 +239,49 @@ def test_operand_store_modes(hip_ctx, oracle):
                 hip_ctx.release(sys_host)
     finally:
         hip_ctx.release(ref)
+
+
+@pytest.mark.parametrize("no,nv,seed,with_j", [
+    (1, 3, 61, False), (3, 5, 62, True), (4, 9, 63, False), (9, 14, 64, True),
+    (17, 10, 65, False),
+])
+def test_complex_against_oracle(hip_ctx, oracle, no, nv, seed, with_j):
+    """Complex (T): device-generated and host-uploaded complex systems
+    against the complex oracle (E, E(cT), per tuple)."""
+    print("complex", no, nv, seed, with_j, flush=True)
+    t = oracle.synthetic_z(no, nv, seed, with_j=with_j)
+    e_ref, ect_ref, pt_ref, ptc_ref = oracle.run_z(no, nv, t, per_tuple=True)
+    s_host = hip_ctx.upload_complex(no, nv, t)
+    s_dev = hip_ctx.upload_synthetic_complex(no, nv, seed, with_j=with_j)
+    try:
+        e, ect = hip_ctx.run(s_host)
+        n = oracle.n_tuples(nv)
+        pt, ptc = hip_ctx.tuple_energies(s_host, 0, n, with_ct=True)
+        assert hip_ctx.run(s_dev) == (e, ect)  # same inputs bit for bit
+        e_ijk, _ = hip_ctx.run(s_dev, ijkabc=True)
+        with pytest.raises(Exception, match="FP64 only"):
+            hip_ctx.run(s_dev, "split_fp32")
+    finally:
+        hip_ctx.release(s_host)
+        hip_ctx.release(s_dev)
+    scale = np.max(np.abs(pt_ref))
+    assert np.max(np.abs(pt - pt_ref)) <= REL_TUPLE * scale + FLOOR
+    assert np.max(np.abs(ptc - ptc_ref)) <= REL_TUPLE * np.max(np.abs(ptc_ref)) + FLOOR
+    assert abs(e - e_ref) <= REL_E * abs(e_ref) + FLOOR
+    assert abs(ect - ect_ref) <= REL_E * abs(ect_ref) + FLOOR
+    e_ijk_ref, _ = oracle.run_z(no, nv, t, ijkabc=True)
+    assert abs(e_ijk - e_ijk_ref) <= REL_E * abs(e_ijk_ref) + FLOOR
+
+
+def test_complex_with_zero_imaginary_is_real(hip_ctx, oracle):
+    no, nv = 6, 11
+    t = oracle.synthetic(no, nv, 70, with_j=True)
+    s_r = hip_ctx.upload(no, nv, t)
+    s_z = hip_ctx.upload_complex(no, nv, {k: v.astype(np.complex128) for k, v in t.items()})
+    try:
+        er, ectr = hip_ctx.run(s_r)
+        ez, ectz = hip_ctx.run(s_z)
+    finally:
+        hip_ctx.release(s_r)
+        hip_ctx.release(s_z)
+    assert abs(ez - er) <= 1e-13 * abs(er) and abs(ectz - ectr) <= 1e-13 * abs(ectr)
