@@ -78,16 +78,24 @@ class Server:
         self.ctx._check(self._L.atrip_hip_serve_step(self._h, C.byref(n)), "serve_step")
         return n.value
 
-    def result(self, rid: int) -> dict:
+    def result(self, rid: int, raise_failed: bool = True) -> dict:
+        """State, energy and times of a request; a failed request raises
+        HipError with its reason (or is returned with state FAILED and the
+        reason under "error" when raise_failed is False)."""
         st, e = C.c_int(), C.c_double()
         ts, ta, td = C.c_double(), C.c_double(), C.c_double()
-        self.ctx._check(self._L.atrip_hip_serve_result(
+        rc = self._L.atrip_hip_serve_result(
             self._h, rid, C.byref(st), C.byref(e), C.byref(ts), C.byref(ta),
-            C.byref(td)), f"serve_result({rid})")
-        if st.value == DONE:
+            C.byref(td))
+        out = dict(state=st.value, energy=e.value, t_submit=ts.value,
+                   t_admit=ta.value, t_done=td.value)
+        if rc != 0:
+            if st.value != FAILED or raise_failed:
+                self.ctx._check(rc, f"serve_result({rid})")
+            out["error"] = self._L.atrip_hip_last_error(self.ctx._h).decode()
+        if st.value in (DONE, FAILED):
             self._keep.pop(rid, None)
-        return dict(state=st.value, energy=e.value, t_submit=ts.value,
-                    t_admit=ta.value, t_done=td.value)
+        return out
 
     def stats(self) -> dict:
         q, a = C.c_int(), C.c_int()
@@ -107,7 +115,7 @@ class Server:
         while pending:
             self.step()
             for rid in list(pending):
-                r = self.result(rid)
+                r = self.result(rid, raise_failed=False)
                 if r["state"] in (DONE, FAILED):
                     out[rid] = r
                     pending.discard(rid)
@@ -142,7 +150,7 @@ def run_poisson(ctx: hip.Context, n_requests: int, rate: float = 20.0,
             for q, rid in enumerate(ids):
                 if rid in done:
                     continue
-                r = srv.result(rid)
+                r = srv.result(rid, raise_failed=False)
                 if r["state"] in (DONE, FAILED):
                     done[rid] = r
                     if log:
@@ -150,8 +158,11 @@ def run_poisson(ctx: hip.Context, n_requests: int, rate: float = 20.0,
                             f"latency {r['t_done'] - r['t_submit']:.0f} ms")
         st = srv.stats()
         t_end = max(r["t_done"] for r in done.values())
-    lat = np.array([r["t_done"] - r["t_submit"] for r in done.values()])
-    queue = np.array([r["t_admit"] - r["t_submit"] for r in done.values()])
+    ok = [r for r in done.values() if r["state"] == DONE]
+    if not ok:
+        raise hip.HipError("no served request finished")
+    lat = np.array([r["t_done"] - r["t_submit"] for r in ok])
+    queue = np.array([r["t_admit"] - r["t_submit"] for r in ok])
     span_s = (t_end - t0) * 1e-3
     flop = sum(workloads.doubles_flop(r[1], r[2]) for r in reqs)
     return dict(
@@ -162,5 +173,7 @@ def run_poisson(ctx: hip.Context, n_requests: int, rate: float = 20.0,
         tuples_per_s=st["tuples_done"] / span_s,
         tflops=flop / span_s / 1e12, busy_fraction=st["busy_ms"] * 1e-3 / span_s,
         launches=st["launches"], peak_resident_gb=st["peak_bytes"] / 1e9,
-        energies={ids.index(k): v["energy"] for k, v in done.items()},
+        failed=len(done) - len(ok),
+        energies={ids.index(k): v["energy"] for k, v in done.items()
+                  if v["state"] == DONE},
     )
