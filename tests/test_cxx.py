@@ -41,3 +41,18 @@ def test_cli_energy_matches_oracle(oracle, tmp_path):
     e = float(re.search(r"^Energy: (\S+)", r.stdout, re.M).group(1))
     e_ref, _ = oracle.run(no, nv, oracle.synthetic(no, nv, seed))
     assert abs(e - e_ref) <= 1e-10 * abs(e_ref)
+
+
+@pytest.mark.gpu
+def test_cli_complex_matches_oracle(oracle, tmp_path):
+    _need(CLI)
+    no, nv, seed = 5, 9, 4
+    r = subprocess.run([CLI, "--no", str(no), "--nv", str(nv), "--seed", str(seed),
+                        "--complex", "--cT", "--nocheckpoint"], capture_output=True,
+                       text=True, timeout=300, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    e = float(re.search(r"^Energy: (\S+)", r.stdout, re.M).group(1))
+    ect = float(re.search(r"^Energy \(cT\): (\S+)", r.stdout, re.M).group(1))
+    e_ref, ect_ref = oracle.run_z(no, nv, oracle.synthetic_z(no, nv, seed, with_j=True))
+    assert abs(e - e_ref) <= 1e-10 * abs(e_ref)
+    assert abs(ect - ect_ref) <= 1e-10 * abs(ect_ref)
