@@ -31,6 +31,11 @@ METRIC = ("(T) tuples/s & TFLOP/s, FP64 vs mixed @|ΔE|≤1e-7 Eh; "
 PEAK_TFLOPS = {"fp64": 78.6, "split_fp32": 157.3, "fp32": 157.3}
 
 
+def log(msg: str):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def n_tuples(nv: int) -> int:
     return nv * (nv + 1) * (nv + 2) // 6 - nv
 
@@ -124,6 +129,7 @@ def bench_batched(ctx, n_systems: int, seed: int):
     t0 = time.time()
     systems = [ctx.upload_synthetic(no, nv, s) for no, nv, s in shapes]
     up_s = time.time() - t0
+    log(f"{n_systems} systems resident; tuning")
     t0 = time.time()
     ctx.batch_tune(systems, measure=True)
     tune_s = time.time() - t0
@@ -157,7 +163,7 @@ def bench_serving(ctx, n_requests: int, rate: float, seed: int):
     from atrip_amd import serve
 
     r = serve.run_poisson(ctx, n_requests, rate=rate, seed=seed,
-                          hbm_budget_gb=200.0, iter_ms=100.0)
+                          hbm_budget_gb=200.0, iter_ms=100.0, log=log)
     r.pop("energies")
     r["workload"] = (f"{n_requests} requests, Poisson {rate} req/s, No log-U[8,40], "
                      f"Nv log-U[40,320], seed {seed}, tensor seeds 5000+r; "
@@ -176,6 +182,7 @@ def bench_c3(ctx, no: int, nv: int, seed: int, a0: int, a1: int, window: int):
                         f"in [{a0},{a1}) ({count})"), "tuples": count}
     ref = None
     for prec, store in (("fp64", 0), ("split_fp32", 1), ("fp32", 2)):
+        log(f"No={no} Nv={nv} sample, {prec}")
         ctx.set_option("operand_store", store)
         try:
             t0 = time.time()
@@ -272,6 +279,7 @@ def main(argv=None):
         e, _ = ctx.run(sys_, args.precision, first=first, count=count)
         e_sum += e
         done += count
+        log(f"step {i + 1}/{args.steps}: {count} tuples")
     t_end = time.perf_counter()
     elapsed = t_end - t_start
     ctx.set_option("timing", 0)
@@ -297,6 +305,7 @@ def main(argv=None):
             first, count = slices[i % len(slices)]
             em, _ = ctx.run(sys_, args.mixed, first=first, count=count)
             m_sum += em
+            log(f"{args.mixed} step {i + 1}/{args.steps}")
         m_elapsed = time.perf_counter() - t0m
         ctx.set_option("timing", 0)
         mk = ctx.kernel_stats("doubles_mixed" if args.mixed == "split_fp32"
@@ -316,7 +325,9 @@ def main(argv=None):
         }
 
     ctx.release(sys_)
+    log("batched form")
     batched = None if args.no_batched else bench_batched(ctx, args.batch_systems, 0)
+    log("serving form")
     serving = None if args.no_serve else bench_serving(ctx, args.serve_requests, 20.0, 0)
     c3 = None
     if not args.no_c3:
@@ -325,6 +336,7 @@ def main(argv=None):
 
     cpu = None
     if not args.no_cpu:
+        log("CPU baseline")
         cores = min(len(os.sched_getaffinity(0)), 16)
         cpu = cpu_baseline(args.no, args.nv, args.seed, args.cpu_seconds, cores)
 
