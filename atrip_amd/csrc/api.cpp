@@ -358,8 +358,8 @@ void complex_negations(atrip_hip_ctx *ctx, DevSystem &s) {
 }
 
 // The four real class-GEMM passes of a complex system: W = Wr + i Wi with
-//   Wr = [T3r V3r + H3r N_r] + [(-T3i) V3i + H3i N_i]   (passes 0, 1)
-//   Wi = [T3r V3i + H3r N_i] + [T3i V3r + (-H3i) N_r]   (passes 2, 3)
+//   Wr = [T3r V3r + H3r N_r] + [(-T3i) V3i + H3i N_i]
+//   Wi = [T3r V3i + H3r N_i] + [T3i V3r + (-H3i) N_r]
 // where N = -T2 (the pre-negated hole operands) and the hole terms carry
 // conj(Vhhh) = H3r - i H3i, as the reference's complex GEMM path does
 // (MAYBE_CONJ, src/atrip/Equations.cxx:620-656).
@@ -512,25 +512,26 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   const int64_t slots_max = (int64_t)std::min(bw, nsel) * nb3;
   const int64_t W3 = w_pad(s.No) * w_pad(s.No) * w_pad(s.No);
   const bool use_j = s.has_j;
-  // W of the batch's slots in FP64 (every precision): one buffer, two for
-  // complex systems (Wr, Wi), twice that with the (cT) pass
-  const size_t gsz = (size_t)(slots_max * W3) * sizeof(double);
-  const int nsets = s.cplx ? 2 : 1;
+  const size_t gsz =
+      (size_t)(3 * slots_max * W3) * (g32 ? sizeof(float) : sizeof(double));
+  // complex: four class-buffer sets per pass (Wr = G0 + G1, Wi = G2 + G3)
+  const int nsets = s.cplx ? 4 : 1;
   ensure((void *&)ctx->G, ctx->G_bytes, (use_j ? 2 : 1) * nsets * gsz,
          "alloc G");
-  ensure((void *&)ctx->e_slot, ctx->e_slot_bytes,
-         (size_t)(2 * slots_max) * sizeof(double), "alloc e_slot");
   DevSystem zpass[4];
   if (s.cplx) complex_passes(s, zpass);
-  double *Gbuf[4];
-  for (int q = 0; q < 4; q++) Gbuf[q] = (double *)((char *)ctx->G + q * gsz);
-  // real: G = Gbuf[0], GJ = Gbuf[1]; complex: Wr, Wi = Gbuf[0], Gbuf[1],
-  // (cT) Wr, Wi = Gbuf[2], Gbuf[3]
+  double *Gz[4], *GJz[4];
+  for (int q = 0; q < 4; q++) {
+    Gz[q] = (double *)((char *)ctx->G + q * gsz);
+    GJz[q] = (double *)((char *)ctx->G + (4 + q) * gsz);
+  }
+  ensure((void *&)ctx->e_slot, ctx->e_slot_bytes,
+         (size_t)(2 * slots_max) * sizeof(double), "alloc e_slot");
   const int nbatch = (nsel + bw - 1) / bw;
   ensure((void *&)ctx->partials, ctx->partials_bytes,
          (size_t)(4 * nbatch) * sizeof(double), "alloc partials");
-  double *G = Gbuf[0];
-  double *GJ = use_j ? Gbuf[1] : nullptr;
+  void *G = ctx->G;
+  void *GJ = use_j ? (void *)((char *)ctx->G + gsz) : nullptr;
   // complex products are four real ones (the reference bench counts 4x)
   const double flop_tuple = 12.0 * (double)s.No * s.No * s.No * (s.No + s.Nv) *
                             (s.cplx ? 4.0 : 1.0);
@@ -557,32 +558,31 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
       check(hipEventCreate(&e2), "event");
       check(hipEventRecord(e0, ctx->stream), "event");
     }
-    // complex: passes 0, 1 accumulate Wr and 2, 3 Wi (see complex_passes)
     for (int pass = 0; pass < (use_j ? 2 : 1) && s.cplx; pass++)
       for (int q = 0; q < 4; q++)
         check(launch_doubles_f64(zpass[q], b, pass == 1,
-                                 Gbuf[2 * pass + q / 2], ctx->tile,
-                                 ctx->stream, (q & 1) != 0),
+                                 pass == 0 ? Gz[q] : GJz[q], ctx->tile,
+                                 ctx->stream),
               "doubles kernel (complex pass)");
     for (int pass = 0; pass < (use_j ? 2 : 1) && !s.cplx; pass++) {
-      double *Gp = pass == 0 ? G : GJ;
+      void *Gp = pass == 0 ? G : GJ;
       if (precision == ATRIP_FP64)
-        check(launch_doubles_f64(s, b, pass == 1, Gp, ctx->tile, ctx->stream),
+        check(launch_doubles_f64(s, b, pass == 1, (double *)Gp, ctx->tile,
+                                 ctx->stream),
               "doubles kernel");
       else
-        check(launch_doubles_split(s, b, pass == 1, Gp,
+        check(launch_doubles_split(s, b, pass == 1, (float *)Gp,
                                    precision == ATRIP_SPLIT_FP32, ctx->tile,
                                    ctx->stream),
               "doubles (FP32 products) kernel");
     }
     if (ctx->timing) check(hipEventRecord(e1, ctx->stream), "event");
     if (s.cplx)
-      check(launch_energy_z(s, b, Gbuf[0], Gbuf[1], use_j ? Gbuf[2] : nullptr,
-                            use_j ? Gbuf[3] : nullptr, ijkabc ? -1.0 : 1.0,
-                            ctx->e_slot, ctx->stream),
+      check(launch_energy_z(s, b, Gz, use_j ? GJz : nullptr,
+                            ijkabc ? -1.0 : 1.0, ctx->e_slot, ctx->stream),
             "energy kernel (complex)");
     else
-      check(launch_energy(s, b, G, GJ, ijkabc ? -1.0 : 1.0, ctx->e_slot,
+      check(launch_energy(s, b, G, GJ, g32, ijkabc ? -1.0 : 1.0, ctx->e_slot,
                           ctx->stream),
             "energy kernel");
     if (ctx->timing) {

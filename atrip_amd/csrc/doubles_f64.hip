@@ -30,8 +30,7 @@ struct DoublesParams {
   const double *T3, *V3, *H3, *T2N, *T2TN;
   const int32_t *windows;
   const uint64_t *aprefix;
-  double *G;       // W of the batch's slots, [slot][W3]
-  int cls, add;    // class of this launch; add into G instead of storing
+  double *G;
   int64_t No, Nv, Kv, Ko, N2, W3, nbk, slots;
   uint64_t first, last;
   int nb, mtiles, ntiles;
@@ -58,7 +57,7 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
   const int mt = (int)(rem % P.mtiles); rem /= P.mtiles;
   const int nb = P.nb;
   const int pair = (int)(rem % (nb * nb)); rem /= (nb * nb);
-  const int cl = P.cls;
+  const int cl = (int)(rem % 3); rem /= 3;
   const int64_t w = rem;
   const int64_t al = P.windows[3 * w], be = P.windows[3 * w + 1],
                 ga = P.windows[3 * w + 2];
@@ -173,7 +172,7 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
 #undef SET_PTRS
   // store: for v_mfma_f64_16x16x4f64, D[i][j] sits in lane j + 16*(i%4),
   // register i/4 (measured on gfx950)
-  double *Gc = P.G + w * (int64_t)nb * nb * nb * P.W3;
+  double *Gc = P.G + ((int64_t)cl * P.slots + w * (int64_t)nb * nb * nb) * P.W3;
 #pragma unroll
   for (int g = 0; g < FN; g++) {
     if (!cvalid[g]) continue;
@@ -189,8 +188,7 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
         const int64_t p = m % No, q = m / No;
         int64_t i, j, k;
         class_out(cl, p, q, rr, i, j, k);
-        double *dst = Gs + w_index(i, j, k, P.nbk);
-        *dst = P.add ? *dst + acc[f][g][r] : acc[f][g][r];
+        Gs[w_index(i, j, k, P.nbk)] = acc[f][g][r];
       }
     }
   }
@@ -199,8 +197,8 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
 }  // namespace
 
 hipError_t launch_doubles_f64(const DevSystem &s, const BatchArgs &b,
-                              bool use_j, double *G, int tile, hipStream_t st,
-                              bool accumulate) {
+                              bool use_j, double *G, int tile,
+                              hipStream_t st) {
   DoublesParams P;
   P.T3 = s.T3;
   P.V3 = use_j ? s.V3J : s.V3;
@@ -226,25 +224,18 @@ hipError_t launch_doubles_f64(const DevSystem &s, const BatchArgs &b,
   const int64_t MT = 32 * FM, NT = 32 * FN;
   P.mtiles = (int)((P.N2 + MT - 1) / MT);
   P.ntiles = (int)(((int64_t)b.nb * s.No + NT - 1) / NT);
-  P.total = (int64_t)b.nw * b.nb * b.nb * P.mtiles * P.ntiles;
+  P.total = (int64_t)b.nw * 3 * b.nb * b.nb * P.mtiles * P.ntiles;
   P.per_xcd = (P.total + 7) / 8;
   const int64_t grid = P.per_xcd * 8;
   if (grid <= 0) return hipSuccess;
-  // one launch per class, in the order w, v, u: W = ((w + v) + u)
-  for (int cl = 0; cl < 3; cl++) {
-    P.cls = cl;
-    P.add = accumulate || cl > 0;
 #define LAUNCH(A, B)                                                       \
   if (FM == A && FN == B)                                                  \
     hipLaunchKernelGGL((doubles_f64_kernel<A, B>), dim3((unsigned)grid),   \
                        dim3(256), 0, st, P);
-    LAUNCH(2, 2) LAUNCH(4, 2) LAUNCH(2, 4) LAUNCH(4, 4) LAUNCH(3, 3)
-    LAUNCH(5, 5) LAUNCH(4, 5) LAUNCH(5, 4)
+  LAUNCH(2, 2) LAUNCH(4, 2) LAUNCH(2, 4) LAUNCH(4, 4) LAUNCH(3, 3)
+  LAUNCH(5, 5) LAUNCH(4, 5) LAUNCH(5, 4)
 #undef LAUNCH
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+  return hipGetLastError();
 }
 
 }  // namespace atrip_amd

@@ -29,8 +29,7 @@ struct SplitParams {
   const __bf16 *T3l, *H3l, *V3l, *T2Nl, *T2TNl;
   const int32_t *windows;
   const uint64_t *aprefix;
-  double *G;       // W of the batch's slots, [slot][W3]
-  int cls, add;    // class of this launch; add into G instead of storing
+  float *G;
   int64_t No, Nv, Kv, Ko, N2, W3, nbk, slots;
   uint64_t first, last;
   int nb, mtiles, ntiles;
@@ -48,7 +47,7 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
   const int mt = (int)(rem % P.mtiles); rem /= P.mtiles;
   const int nb = P.nb;
   const int pair = (int)(rem % (nb * nb)); rem /= (nb * nb);
-  const int cl = P.cls;
+  const int cl = (int)(rem % 3); rem /= 3;
   const int64_t w = rem;
   const int64_t al = P.windows[3 * w], be = P.windows[3 * w + 1],
                 ga = P.windows[3 * w + 2];
@@ -191,11 +190,11 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
 
   // store: for the 16x16 FP32-output MFMAs, D[i][j] sits in lane
   // j + 16*(i/4), register i%4 (measured on gfx950)
-  double *Gc = P.G + w * (int64_t)nb * nb * nb * P.W3;
+  float *Gc = P.G + ((int64_t)cl * P.slots + w * (int64_t)nb * nb * nb) * P.W3;
 #pragma unroll
   for (int g = 0; g < FN; g++) {
     if (!cvalid[g]) continue;
-    double *Gs = Gc + cslot[g] * P.W3;
+    float *Gs = Gc + cslot[g] * P.W3;
     const int64_t rr = crr[g];
 #pragma unroll
     for (int f = 0; f < FM; f++) {
@@ -206,9 +205,7 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
         if (m >= N2) continue;
         int64_t i, j, k;
         class_out(cl, m % No, m / No, rr, i, j, k);
-        double *dst = Gs + w_index(i, j, k, P.nbk);
-        const double v = (double)acc[f][g][r];
-        *dst = P.add ? *dst + v : v;
+        Gs[w_index(i, j, k, P.nbk)] = acc[f][g][r];
       }
     }
   }
@@ -238,7 +235,7 @@ hipError_t launch_split(const double *x, DevSystem::Split &out, int64_t n,
 }
 
 hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
-                                bool use_j, double *G, bool corrections,
+                                bool use_j, float *G, bool corrections,
                                 int tile, hipStream_t st) {
   SplitParams P;
   const DevSystem::Split &V = use_j ? s.sV3J : s.sV3;
@@ -270,14 +267,10 @@ hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
   const int64_t MT = 32 * FM, NT = 32 * FN;
   P.mtiles = (int)((P.N2 + MT - 1) / MT);
   P.ntiles = (int)(((int64_t)b.nb * s.No + NT - 1) / NT);
-  P.total = (int64_t)b.nw * b.nb * b.nb * P.mtiles * P.ntiles;
+  P.total = (int64_t)b.nw * 3 * b.nb * b.nb * P.mtiles * P.ntiles;
   P.per_xcd = (P.total + 7) / 8;
   const int64_t grid = P.per_xcd * 8;
   if (grid <= 0) return hipSuccess;
-  // one launch per class, in the order w, v, u: W = ((w + v) + u) in FP64
-  for (int cl = 0; cl < 3; cl++) {
-    P.cls = cl;
-    P.add = cl > 0;
 #define LAUNCH(A, B)                                                         \
   if (FM == A && FN == B) {                                                  \
     if (corrections)                                                         \
@@ -287,13 +280,10 @@ hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
       hipLaunchKernelGGL((doubles_split_kernel<A, B, false>),                \
                          dim3((unsigned)grid), dim3(256), 0, st, P);         \
   }
-    LAUNCH(2, 2) LAUNCH(4, 2) LAUNCH(2, 4) LAUNCH(4, 4) LAUNCH(3, 3)
-    LAUNCH(5, 5) LAUNCH(4, 5) LAUNCH(5, 4)
+  LAUNCH(2, 2) LAUNCH(4, 2) LAUNCH(2, 4) LAUNCH(4, 4) LAUNCH(3, 3)
+  LAUNCH(5, 5) LAUNCH(4, 5) LAUNCH(5, 4)
 #undef LAUNCH
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+  return hipGetLastError();
 }
 
 }  // namespace atrip_amd

@@ -1,10 +1,11 @@
 // Singles + energy epilogue of complex (T) (atrip::Atrip::run<Complex>).
 //
 // The doubles of a complex system run as four real class-GEMM passes on
-// real/imaginary operand planes (see the run loop), two accumulating Wr and
-// two Wi.  Per tuple (one workgroup), for each block triple of edge 8 the
-// cubes holding W at the six index permutations are copied into LDS as
-// (re, im), Z = W + singles is formed with complex
+// real/imaginary operand planes (see the run loop), so W = Wr + i Wi arrives
+// as four sets of class buffers: Wr = G0 + G1, Wi = G2 + G3 (each the sum of
+// its three classes w, v, u).  Per tuple (one workgroup), for each block
+// triple of edge 8 the cubes holding W at the six index permutations are
+// summed into LDS as (re, im), Z = W + singles is formed with complex
 // products (reference src/atrip/Equations.cxx:387-426), and the energy
 // follows get_energy_distinct / get_energy_same (:101-238) with F = Complex:
 // A..F = conj(W) (maybe_conjugate_scalar), U..Z = Z, complex denominators,
@@ -37,7 +38,7 @@ __device__ inline double re_div(cplx a, cplx b) {
 }
 
 struct EnergyZParams {
-  const double *Gr, *Gi, *GJr, *GJi;  // W (and the (cT) pass's), [slot][W3]
+  const double *G[4], *GJ[4];  // class buffer sets: Wr = G0 + G1, Wi = G2 + G3
   const double *Tair, *Taii, *VHHr, *VHHi, *eir, *eii, *ear, *eai;
   const int32_t *windows;
   const uint64_t *aprefix;
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void energy_z_kernel(EnergyZParams P) {
   auto vhh = [&](int64_t base, int p, int q) {
     return cplx{P.VHHr[base + p + no * q], P.VHHi[base + p + no * q]};
   };
-  const int64_t off = gslot * P.W3;
+  const int64_t off = gslot * P.W3, cls = P.slots * P.W3;
 
   double e_acc = 0.0, ect_acc = 0.0;
 #pragma unroll 1
@@ -112,11 +113,19 @@ __global__ __launch_bounds__(256) void energy_z_kernel(EnergyZParams P) {
           const int64_t base = off + (((int64_t)(cz[p] * nbk + cy[p]) * nbk + cx[p]) << 9);
           for (int q = tid; q < 512; q += 256) {
             const int64_t e = base + q;
-            cr[p][q] = P.Gr[e];
-            ci[p][q] = P.Gi[e];
+            const double r0 = (P.G[0][e] + P.G[0][e + cls]) + P.G[0][e + 2 * cls];
+            const double r1 = (P.G[1][e] + P.G[1][e + cls]) + P.G[1][e + 2 * cls];
+            const double i0 = (P.G[2][e] + P.G[2][e + cls]) + P.G[2][e + 2 * cls];
+            const double i1 = (P.G[3][e] + P.G[3][e + cls]) + P.G[3][e + 2 * cls];
+            cr[p][q] = r0 + r1;
+            ci[p][q] = i0 + i1;
             if constexpr (HJ) {
-              jr[p][q] = P.GJr[e];
-              ji[p][q] = P.GJi[e];
+              const double s0 = (P.GJ[0][e] + P.GJ[0][e + cls]) + P.GJ[0][e + 2 * cls];
+              const double s1 = (P.GJ[1][e] + P.GJ[1][e + cls]) + P.GJ[1][e + 2 * cls];
+              const double t0 = (P.GJ[2][e] + P.GJ[2][e + cls]) + P.GJ[2][e + 2 * cls];
+              const double t1 = (P.GJ[3][e] + P.GJ[3][e + cls]) + P.GJ[3][e + 2 * cls];
+              jr[p][q] = s0 + s1;
+              ji[p][q] = t0 + t1;
             }
           }
         }
@@ -190,14 +199,13 @@ __global__ __launch_bounds__(256) void energy_z_kernel(EnergyZParams P) {
 }  // namespace
 
 hipError_t launch_energy_z(const DevSystem &s, const BatchArgs &b,
-                           const double *Gr, const double *Gi,
-                           const double *GJr, const double *GJi,
+                           double *const G[4], double *const GJ[4],
                            double tai_sign, double *e_slot, hipStream_t st) {
   EnergyZParams P;
-  P.Gr = Gr;
-  P.Gi = Gi;
-  P.GJr = GJr;
-  P.GJi = GJi;
+  for (int q = 0; q < 4; q++) {
+    P.G[q] = G[q];
+    P.GJ[q] = GJ ? GJ[q] : nullptr;
+  }
   P.Tair = s.Tai;
   P.Taii = s.Taii;
   P.VHHr = s.VHH;
@@ -220,7 +228,7 @@ hipError_t launch_energy_z(const DevSystem &s, const BatchArgs &b,
   P.nb = b.nb;
   P.tai_sign = tai_sign;
   const dim3 grid((unsigned)b.slots), block(256);
-  if (GJr)
+  if (GJ)
     hipLaunchKernelGGL(energy_z_kernel<true>, grid, block, 0, st, P);
   else
     hipLaunchKernelGGL(energy_z_kernel<false>, grid, block, 0, st, P);

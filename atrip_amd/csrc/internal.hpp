@@ -173,30 +173,27 @@ struct BatchArgs {
   int64_t slots;           // nw * nb^3
 };
 
-// Doubles contractions in FP64 MFMA: W[slots][W3] of the batch, one launch
-// per class (w stores, v and u add; accumulate: w adds too).
+// Doubles contractions in FP64 MFMA: class buffers G[3][slots][No^3].
 hipError_t launch_doubles_f64(const DevSystem &s, const BatchArgs &b,
-                              bool use_j, double *G, int tile, hipStream_t st,
-                              bool accumulate = false);
+                              bool use_j, double *G, int tile,
+                              hipStream_t st);
 
-// Doubles contractions with FP32 products and FP32 accumulation, each class
-// tile widened to FP64 into W[slots][W3]; corrections = true adds the
-// hi*lo + lo*hi terms (split FP32).
+// Doubles contractions in split FP32.
+// Doubles contractions with FP32 products (class buffers in FP32):
+// corrections = true adds the hi*lo + lo*hi terms (split FP32).
 hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
-                                bool use_j, double *G, bool corrections,
+                                bool use_j, float *G, bool corrections,
                                 int tile, hipStream_t st);
 
-// Singles + energy epilogue from W (and W of the (cT) pass, or null);
-// writes e_slot[2*slot + {0,1}].
+// Singles + energy epilogue; writes e_slot[2*slot + {0,1}].
 hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
-                         const double *G, const double *GJ, double tai_sign,
-                         double *e_slot, hipStream_t st);
+                         const void *G, const void *GJ, bool g32,
+                         double tai_sign, double *e_slot, hipStream_t st);
 
-// Singles + energy of complex systems from Wr, Wi (and the (cT) pass's,
-// or nulls).
+// Singles + energy of complex systems from the four class-buffer sets of
+// the real passes (Wr = G0 + G1, Wi = G2 + G3); GJ null without (cT).
 hipError_t launch_energy_z(const DevSystem &s, const BatchArgs &b,
-                           const double *Gr, const double *Gi,
-                           const double *GJr, const double *GJi,
+                           double *const G[4], double *const GJ[4],
                            double tai_sign, double *e_slot, hipStream_t st);
 
 // Deterministic compensated sum of the valid slot energies of a batch into
