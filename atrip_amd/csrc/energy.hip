@@ -68,6 +68,9 @@ void energy_kernel(EnergyParams P) {
   __shared__ double cube[6][512];
   __shared__ double cubeJ[HJ ? 6 : 1][HJ ? 512 : 1];
   __shared__ double tA[64], tB[64], tC[64];  // No <= 64 handled in tiles below
+  // V^bc, V^ac, V^ab on the 8x8 block pairs (r, s) of the current triple
+  // (positions r != s of blocks B1, B2, B3): sV[slice][r][s][ly + 8 lz]
+  __shared__ double sV[3][3][3][64];
   __shared__ double red[2][4];
 
   const double *Vbc = P.VHH + (b + Nv * c) * N2;
@@ -123,6 +126,28 @@ void energy_kernel(EnergyParams P) {
           tC[tid] = i < No ? P.tai_sign * P.Tai[c + Nv * i] : 0.0;
           tC[8 + tid] = j < No ? P.tai_sign * P.Tai[c + Nv * j] : 0.0;
           tC[16 + tid] = k < No ? P.tai_sign * P.Tai[c + Nv * k] : 0.0;
+        }
+        // doubles-integral blocks of the singles terms for this triple
+        {
+          const int Bp[3] = {B1, B2, B3};
+          double vv[7];
+#pragma unroll
+          for (int it = 0; it < 7; it++) {
+            const int e = tid + 256 * it;  // < 3 * 9 * 64
+            vv[it] = 0.0;
+            if (e < 3 * 9 * 64) {
+              const int v = e / 576, rs = (e / 64) % 9, l = e & 63;
+              const int r = rs / 3, sp = rs % 3;
+              const int y = Bp[r] * 8 + (l & 7), z = Bp[sp] * 8 + (l >> 3);
+              const double *V = v == 0 ? Vbc : v == 1 ? Vac : Vab;
+              if (r != sp && y < No && z < No) vv[it] = V[y + No * z];
+            }
+          }
+#pragma unroll
+          for (int it = 0; it < 7; it++) {
+            const int e = tid + 256 * it;
+            if (e < 3 * 9 * 64) (&sV[0][0][0][0])[e] = vv[it];
+          }
         }
         // distinct cubes of this orbit, loaded as one flat coalesced sweep
         int cbase[6];
@@ -195,10 +220,13 @@ void energy_kernel(EnergyParams P) {
                     o_kij = lk + 8 * li + 64 * lj, o_kji = lk + 8 * lj + 64 * li;
           // singles at a permutation (x,y,z): t_a[x] V^bc_yz + t_b[y] V^ac_xz
           // + t_c[z] V^ab_xy; block slots: 0 -> i, 8 -> j, 16 -> k
-          auto Z = [&](double wv, int sx, int sy, int sz, int x, int y, int z) {
-            double zz = wv + tA[sx] * Vbc[y + no * z];
-            zz = zz + tB[sy] * Vac[x + no * z];
-            zz = zz + tC[sz] * Vab[x + no * y];
+          // slot s = 8 * position + local index (position 0: i, 1: j, 2: k)
+          auto Z = [&](double wv, int sx, int sy, int sz, int, int, int) {
+            const int px = sx >> 3, py = sy >> 3, pz = sz >> 3;
+            const int lx = sx & 7, ly = sy & 7, lz = sz & 7;
+            double zz = wv + tA[sx] * sV[0][py][pz][ly + 8 * lz];
+            zz = zz + tB[sy] * sV[1][px][pz][lx + 8 * lz];
+            zz = zz + tC[sz] * sV[2][px][py][lx + 8 * ly];
             return zz;
           };
           const int si = li, sj = 8 + lj, sk = 16 + lk;
