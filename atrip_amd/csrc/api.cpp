@@ -510,7 +510,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   const int bw = std::max(1, ctx->batch);
   const int64_t nb3 = (int64_t)nb * nb * nb;
   const int64_t slots_max = (int64_t)std::min(bw, nsel) * nb3;
-  const int64_t W3 = w_pad(s.No) * w_pad(s.No) * w_pad(s.No);
+  const int64_t W3 = s.No * s.No * s.No;
   const bool use_j = s.has_j;
   const size_t gsz =
       (size_t)(3 * slots_max * W3) * (g32 ? sizeof(float) : sizeof(double));

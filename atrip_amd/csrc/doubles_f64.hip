@@ -31,7 +31,7 @@ struct DoublesParams {
   const int32_t *windows;
   const uint64_t *aprefix;
   double *G;
-  int64_t No, Nv, Kv, Ko, N2, W3, nbk, slots;
+  int64_t No, Nv, Kv, Ko, N2, W3, slots;
   uint64_t first, last;
   int nb, mtiles, ntiles;
   int64_t total, per_xcd;
@@ -185,10 +185,7 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
         const int64_t m =
             (int64_t)mt * MT + wm * (MT / 2) + f * 16 + 4 * r + kk;
         if (m >= N2) continue;
-        const int64_t p = m % No, q = m / No;
-        int64_t i, j, k;
-        class_out(cl, p, q, rr, i, j, k);
-        Gs[w_index(i, j, k, P.nbk)] = acc[f][g][r];
+        Gs[m * No + rr] = acc[f][g][r];  // GEMM order: 128-byte rows
       }
     }
   }
@@ -213,8 +210,7 @@ hipError_t launch_doubles_f64(const DevSystem &s, const BatchArgs &b,
   P.Kv = s.Kv;
   P.Ko = s.Ko;
   P.N2 = s.No * s.No;
-  P.nbk = w_pad(s.No) / WCUBE;
-  P.W3 = w_pad(s.No) * w_pad(s.No) * w_pad(s.No);
+  P.W3 = s.No * s.No * s.No;
   P.slots = b.slots;
   P.first = b.first;
   P.last = b.last;

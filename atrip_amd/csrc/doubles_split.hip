@@ -30,7 +30,7 @@ struct SplitParams {
   const int32_t *windows;
   const uint64_t *aprefix;
   float *G;
-  int64_t No, Nv, Kv, Ko, N2, W3, nbk, slots;
+  int64_t No, Nv, Kv, Ko, N2, W3, slots;
   uint64_t first, last;
   int nb, mtiles, ntiles;
   int64_t total, per_xcd;
@@ -203,9 +203,7 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
         const int64_t m =
             (int64_t)mt * MT + wm * (MT / 2) + f * 16 + 4 * kk + r;
         if (m >= N2) continue;
-        int64_t i, j, k;
-        class_out(cl, m % No, m / No, rr, i, j, k);
-        Gs[w_index(i, j, k, P.nbk)] = acc[f][g][r];
+        Gs[m * No + rr] = acc[f][g][r];  // GEMM order: 64-byte rows
       }
     }
   }
@@ -256,8 +254,7 @@ hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
   P.Kv = s.Kv;
   P.Ko = s.Ko;
   P.N2 = s.No * s.No;
-  P.nbk = w_pad(s.No) / WCUBE;
-  P.W3 = w_pad(s.No) * w_pad(s.No) * w_pad(s.No);
+  P.W3 = s.No * s.No * s.No;
   P.slots = b.slots;
   P.first = b.first;
   P.last = b.last;

@@ -1,9 +1,9 @@
 // Fused singles + energy epilogue of the (T) hot path.
 //
 // One workgroup per tuple slot of a window batch.  W is the sum of the three
-// class buffers written by the doubles kernel (cube-blocked layout, see
-// w_index; FP64, or FP32 for the single-precision-product forms, widened to
-// FP64 on load), Z = W + singles is formed in registers (reference src/atrip/Equations.cxx:387-426, with Z copied from W
+// class buffers written by the doubles kernel (each in its GEMM output order,
+// see internal.hpp; FP64, or FP32 for the single-precision-product forms,
+// widened to FP64 on load), Z = W + singles is formed in registers (reference src/atrip/Equations.cxx:387-426, with Z copied from W
 // as in src/atrip/Atrip.cxx:899-906), and the energy of the tuple follows
 // get_energy_distinct / get_energy_same (src/atrip/Equations.cxx:101-238)
 // selected as src/atrip/Atrip.cxx:636-670 does.  The i>=j>=k points are
@@ -21,7 +21,7 @@ struct EnergyParams {
   const int32_t *windows;
   const uint64_t *aprefix;
   double *e_slot;
-  int64_t No, Nv, N2, W3, nbk, slots;
+  int64_t No, Nv, N2, W3, nbk, slots;  // nbk: 8-blocks per occupied index
   uint64_t first, last;
   int nb;
   double tai_sign;
@@ -150,59 +150,70 @@ void energy_kernel(EnergyParams P) {
           }
         }
         // distinct cubes of this orbit, loaded as one flat coalesced sweep
-        int cbase[6];
+        int cxs[6], cys[6], czs[6];
         int nc = 0, slotof[6];
 #pragma unroll
         for (int p = 0; p < 6; p++) {
           slotof[p] = p;
           if (alias[p] == p) {
-            cbase[nc] = ((cz[p] * nbk + cy[p]) * nbk + cx[p]) << 9;
+            cxs[nc] = cx[p] * 8;
+            cys[nc] = cy[p] * 8;
+            czs[nc] = cz[p] * 8;
             slotof[p] = nc++;
           }
         }
 #pragma unroll
         for (int p = 0; p < 6; p++) alias[p] = slotof[alias[p]];
-        // every load of this thread is issued before the first is used
-        // (2 elements per cube per thread), so the cube fetch costs one
-        // memory latency instead of one per element
-        {
+        // The distinct cubes are summed (w + v) + u in LDS in three phases.
+        // Each class is read with the thread order that follows its
+        // fastest index (class w: z, v: y, u: x), so every 8 threads read
+        // 8 consecutive elements; all loads of a thread are issued before
+        // the first is used (2 elements per cube per class).
+        auto stage = [&](const GT *Cw, const GT *Cv, const GT *Cu,
+                         double (*dst)[512]) {
           GT vw[12], vv[12], vu[12];
 #pragma unroll
           for (int it = 0; it < 12; it++) {
             const int p = it >> 1, q = tid + 256 * (it & 1);
+            const int l0 = q & 7, l1 = (q >> 3) & 7, l2 = q >> 6;
+            vw[it] = vv[it] = vu[it] = (GT)0;
             if (p < nc) {
-              const int base = cbase[p];
-              vw[it] = Gw[base + q];
-              vv[it] = Gv[base + q];
-              vu[it] = Gu[base + q];
+              {  // w: (x + No y) No + z, z fastest: (z, y, x) = (l0, l1, l2)
+                const int x = cxs[p] + l2, y = cys[p] + l1, z = czs[p] + l0;
+                if (x < no && y < no && z < no) vw[it] = Cw[(x + no * y) * No + z];
+              }
+              {  // v: (x + No z) No + y, y fastest: (y, x, z) = (l0, l1, l2)
+                const int x = cxs[p] + l1, y = cys[p] + l0, z = czs[p] + l2;
+                if (x < no && y < no && z < no) vv[it] = Cv[(x + no * z) * No + y];
+              }
+              {  // u: (y + No z) No + x, x fastest: (x, y, z) = (l0, l1, l2)
+                const int x = cxs[p] + l0, y = cys[p] + l1, z = czs[p] + l2;
+                if (x < no && y < no && z < no) vu[it] = Cu[(y + no * z) * No + x];
+              }
             }
           }
 #pragma unroll
           for (int it = 0; it < 12; it++) {
             const int p = it >> 1, q = tid + 256 * (it & 1);
-            if (p < nc)
-              cube[p][q] = ((double)vw[it] + (double)vv[it]) + (double)vu[it];
+            const int l0 = q & 7, l1 = (q >> 3) & 7, l2 = q >> 6;
+            if (p < nc) dst[p][l2 + 8 * l1 + 64 * l0] = (double)vw[it];
           }
-          if constexpr (HJ) {
+          __syncthreads();
 #pragma unroll
-            for (int it = 0; it < 12; it++) {
-              const int p = it >> 1, q = tid + 256 * (it & 1);
-              if (p < nc) {
-                const int base = cbase[p];
-                vw[it] = Jw[base + q];
-                vv[it] = Jv[base + q];
-                vu[it] = Ju[base + q];
-              }
-            }
-#pragma unroll
-            for (int it = 0; it < 12; it++) {
-              const int p = it >> 1, q = tid + 256 * (it & 1);
-              if (p < nc)
-                cubeJ[p][q] =
-                    ((double)vw[it] + (double)vv[it]) + (double)vu[it];
-            }
+          for (int it = 0; it < 12; it++) {
+            const int p = it >> 1, q = tid + 256 * (it & 1);
+            const int l0 = q & 7, l1 = (q >> 3) & 7, l2 = q >> 6;
+            if (p < nc) dst[p][l1 + 8 * l0 + 64 * l2] += (double)vv[it];
           }
-        }
+          __syncthreads();
+#pragma unroll
+          for (int it = 0; it < 12; it++) {
+            const int p = it >> 1, q = tid + 256 * (it & 1);
+            if (p < nc) dst[p][q] += (double)vu[it];
+          }
+        };
+        stage(Gw, Gv, Gu, cube);
+        if constexpr (HJ) stage(Jw, Jv, Ju, cubeJ);
         __syncthreads();
 #pragma unroll 1
         for (int pt = tid; pt < 512; pt += 256) {
@@ -355,8 +366,8 @@ hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
   P.No = s.No;
   P.Nv = s.Nv;
   P.N2 = s.No * s.No;
-  P.nbk = w_pad(s.No) / WCUBE;
-  P.W3 = w_pad(s.No) * w_pad(s.No) * w_pad(s.No);
+  P.nbk = w_blocks(s.No);
+  P.W3 = s.No * s.No * s.No;
   P.slots = b.slots;
   P.first = b.first;
   P.last = b.last;

@@ -2,8 +2,8 @@
 //
 // The doubles of a complex system run as four real class-GEMM passes on
 // real/imaginary operand planes (see the run loop), so W = Wr + i Wi arrives
-// as four sets of class buffers: Wr = G0 + G1, Wi = G2 + G3 (each the sum of
-// its three classes w, v, u).  Per tuple (one workgroup), for each block
+// as four sets of class buffers (GEMM output order, see internal.hpp):
+// Wr = G0 + G1, Wi = G2 + G3, each the sum of its three classes w, v, u.  Per tuple (one workgroup), for each block
 // triple of edge 8 the cubes holding W at the six index permutations are
 // summed into LDS as (re, im), Z = W + singles is formed with complex
 // products (reference src/atrip/Equations.cxx:387-426), and the energy
@@ -43,7 +43,7 @@ struct EnergyZParams {
   const int32_t *windows;
   const uint64_t *aprefix;
   double *e_slot;
-  int64_t No, Nv, N2, W3, nbk, slots;
+  int64_t No, Nv, N2, W3, nbk, slots;  // nbk: 8-blocks per occupied index
   uint64_t first, last;
   int nb;
   double tai_sign;
@@ -110,20 +110,29 @@ __global__ __launch_bounds__(256) void energy_z_kernel(EnergyZParams P) {
         const int cz[6] = {B3, B2, B3, B1, B2, B1};
         __syncthreads();
         for (int p = 0; p < 6; p++) {
-          const int64_t base = off + (((int64_t)(cz[p] * nbk + cy[p]) * nbk + cx[p]) << 9);
           for (int q = tid; q < 512; q += 256) {
-            const int64_t e = base + q;
-            const double r0 = (P.G[0][e] + P.G[0][e + cls]) + P.G[0][e + 2 * cls];
-            const double r1 = (P.G[1][e] + P.G[1][e + cls]) + P.G[1][e + 2 * cls];
-            const double i0 = (P.G[2][e] + P.G[2][e + cls]) + P.G[2][e + 2 * cls];
-            const double i1 = (P.G[3][e] + P.G[3][e + cls]) + P.G[3][e + 2 * cls];
+            const int x = cx[p] * 8 + (q & 7), y = cy[p] * 8 + ((q >> 3) & 7),
+                      z = cz[p] * 8 + (q >> 6);
+            double r0 = 0, r1 = 0, i0 = 0, i1 = 0, s0 = 0, s1 = 0, t0 = 0, t1 = 0;
+            if (x < no && y < no && z < no) {
+              // class buffers in GEMM order: w, v, u
+              const int64_t ew = off + (x + No * y) * No + z;
+              const int64_t ev = off + cls + (x + No * z) * No + y;
+              const int64_t eu = off + 2 * cls + (y + No * z) * No + x;
+              r0 = (P.G[0][ew] + P.G[0][ev]) + P.G[0][eu];
+              r1 = (P.G[1][ew] + P.G[1][ev]) + P.G[1][eu];
+              i0 = (P.G[2][ew] + P.G[2][ev]) + P.G[2][eu];
+              i1 = (P.G[3][ew] + P.G[3][ev]) + P.G[3][eu];
+              if constexpr (HJ) {
+                s0 = (P.GJ[0][ew] + P.GJ[0][ev]) + P.GJ[0][eu];
+                s1 = (P.GJ[1][ew] + P.GJ[1][ev]) + P.GJ[1][eu];
+                t0 = (P.GJ[2][ew] + P.GJ[2][ev]) + P.GJ[2][eu];
+                t1 = (P.GJ[3][ew] + P.GJ[3][ev]) + P.GJ[3][eu];
+              }
+            }
             cr[p][q] = r0 + r1;
             ci[p][q] = i0 + i1;
             if constexpr (HJ) {
-              const double s0 = (P.GJ[0][e] + P.GJ[0][e + cls]) + P.GJ[0][e + 2 * cls];
-              const double s1 = (P.GJ[1][e] + P.GJ[1][e + cls]) + P.GJ[1][e + 2 * cls];
-              const double t0 = (P.GJ[2][e] + P.GJ[2][e + cls]) + P.GJ[2][e + 2 * cls];
-              const double t1 = (P.GJ[3][e] + P.GJ[3][e + cls]) + P.GJ[3][e + 2 * cls];
               jr[p][q] = s0 + s1;
               ji[p][q] = t0 + t1;
             }
@@ -220,8 +229,8 @@ hipError_t launch_energy_z(const DevSystem &s, const BatchArgs &b,
   P.No = s.No;
   P.Nv = s.Nv;
   P.N2 = s.No * s.No;
-  P.nbk = w_pad(s.No) / WCUBE;
-  P.W3 = w_pad(s.No) * w_pad(s.No) * w_pad(s.No);
+  P.nbk = w_blocks(s.No);
+  P.W3 = s.No * s.No * s.No;
   P.slots = b.slots;
   P.first = b.first;
   P.last = b.last;

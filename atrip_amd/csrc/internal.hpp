@@ -19,19 +19,16 @@ constexpr int KCHUNK = 16;
 inline int64_t pad16(int64_t n) { return (n + 15) / 16 * 16; }
 
 // Layout of the per-tuple class buffers written by the doubles kernels and
-// read by the energy kernel: W(i,j,k) of one tuple is stored as cubes of edge
-// 8 (512 contiguous doubles each, i fastest inside a cube, cubes ordered
-// i-block fastest), over an occupied range padded to a multiple of 8.  A
-// cube is one coalesced 4 KiB read, which is what the energy kernel's
-// permutation gathers need.
+// read by the energy kernels: each class keeps its GEMM output order,
+// element (row m, column index r) at m * No + r, so a 16-column MFMA output
+// fragment row is one contiguous 128-byte store.  In W indices:
+//   class w: m = i + No j, r = k   ->  (i + No j) No + k
+//   class v: m = i + No k, r = j   ->  (i + No k) No + j
+//   class u: m = j + No k, r = i   ->  (j + No k) No + i
+// The energy kernels stage 8x8x8 blocks of W in LDS (edge WCUBE).
 constexpr int WCUBE = 8;
-__host__ __device__ inline int64_t w_pad(int64_t No) {
-  return (No + WCUBE - 1) / WCUBE * WCUBE;
-}
-__host__ __device__ inline int64_t w_index(int64_t i, int64_t j, int64_t k,
-                                           int64_t nbk) {
-  return ((((k >> 3) * nbk + (j >> 3)) * nbk + (i >> 3)) << 9) + (i & 7) +
-         ((j & 7) << 3) + ((k & 7) << 6);
+__host__ __device__ inline int64_t w_blocks(int64_t No) {
+  return (No + WCUBE - 1) / WCUBE;
 }
 
 // Device-resident system.  Pair-indexed arrays use the CTF pair order
