@@ -527,6 +527,12 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   }
   ensure((void *&)ctx->e_slot, ctx->e_slot_bytes,
          (size_t)(2 * slots_max) * sizeof(double), "alloc e_slot");
+  // W as cubes for the energy kernels: W (and W of the (cT) pass), real and
+  // imaginary parts for complex systems
+  ensure((void *&)ctx->wcube, ctx->wcube_bytes,
+         (size_t)((use_j ? 2 : 1) * (s.cplx ? 2 : 1) * slots_max * w_cube(s.No)) *
+             sizeof(double),
+         "alloc W cubes");
   const int nbatch = (nsel + bw - 1) / bw;
   ensure((void *&)ctx->partials, ctx->partials_bytes,
          (size_t)(4 * nbatch) * sizeof(double), "alloc partials");
@@ -579,11 +585,12 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
     if (ctx->timing) check(hipEventRecord(e1, ctx->stream), "event");
     if (s.cplx)
       check(launch_energy_z(s, b, Gz, use_j ? GJz : nullptr,
-                            ijkabc ? -1.0 : 1.0, ctx->e_slot, ctx->stream),
+                            ijkabc ? -1.0 : 1.0, ctx->e_slot, ctx->wcube,
+                            ctx->stream),
             "energy kernel (complex)");
     else
       check(launch_energy(s, b, G, GJ, g32, ijkabc ? -1.0 : 1.0, ctx->e_slot,
-                          ctx->stream),
+                          ctx->wcube, ctx->stream),
             "energy kernel");
     if (ctx->timing) {
       check(hipEventRecord(e2, ctx->stream), "event");
@@ -701,6 +708,7 @@ int atrip_hip_destroy(atrip_hip_ctx *ctx) {
   if (ctx->e_slot) (void)hipFree(ctx->e_slot);
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->bscratch) (void)hipFree(ctx->bscratch);
+  if (ctx->wcube) (void)hipFree(ctx->wcube);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;

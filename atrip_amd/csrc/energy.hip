@@ -1,9 +1,14 @@
 // Fused singles + energy epilogue of the (T) hot path.
 //
-// One workgroup per tuple slot of a window batch.  W is the sum of the three
-// class buffers written by the doubles kernel (each in its GEMM output order,
-// see internal.hpp; FP64, or FP32 for the single-precision-product forms,
-// widened to FP64 on load), Z = W + singles is formed in registers (reference src/atrip/Equations.cxx:387-426, with Z copied from W
+// wsum_kernel first sums the three class buffers the doubles kernel wrote
+// (each in its GEMM output order, see internal.hpp; FP64, or FP32 for the
+// single-precision-product forms) into W in FP64, (w + v) + u, re-laid out
+// as 8x8x8 cubes (w_cube).  It works on 16x16x16 bricks staged in LDS, so
+// every class is read along its own fastest index in full 128-byte lines and
+// every cube is written as one 4 KiB run.
+//
+// energy_kernel: one workgroup per tuple slot of a window batch.  Z = W +
+// singles is formed in registers (reference src/atrip/Equations.cxx:387-426, with Z copied from W
 // as in src/atrip/Atrip.cxx:899-906), and the energy of the tuple follows
 // get_energy_distinct / get_energy_same (src/atrip/Equations.cxx:101-238)
 // selected as src/atrip/Atrip.cxx:636-670 does.  The i>=j>=k points are
@@ -16,12 +21,12 @@ namespace atrip_amd {
 namespace {
 
 struct EnergyParams {
-  const void *G, *GJ;  // class buffers, element type = template GT
+  const double *W, *WJ;  // W (and the (cT) pass's W) as cubes, [slot][W3]
   const double *Tai, *VHH, *ei, *ea;
   const int32_t *windows;
   const uint64_t *aprefix;
   double *e_slot;
-  int64_t No, Nv, N2, W3, nbk, slots;  // nbk: 8-blocks per occupied index
+  int64_t No, Nv, N2, W3, nbk, slots;  // nbk: cubes per index, W3 = (8 nbk)^3
   uint64_t first, last;
   int nb;
   double tai_sign;
@@ -34,12 +39,90 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// Class sums of one 16x16x16 brick of W for every slot of the batch:
+// out = sum(A) [+ sum(B)], sum(X) = (Xw + Xv) + Xu, written as cubes.
+template <typename GT>
+__global__ __launch_bounds__(256) void wsum_kernel(const GT *A, const GT *B,
+                                                   double *out, int64_t No,
+                                                   int64_t W3g, int64_t slots,
+                                                   int nbk, int nbr) {
+  // LDS brick [z][y][x] with pitch 17 (conflict-free transposed writes)
+  __shared__ double br[16 * 17 * 17];
+  const int64_t bid = blockIdx.x;
+  const int64_t slot = bid / ((int64_t)nbr * nbr * nbr);
+  const int brick = (int)(bid % ((int64_t)nbr * nbr * nbr));
+  const int bx = brick % nbr, by = (brick / nbr) % nbr, bz = brick / (nbr * nbr);
+  const int tid = threadIdx.x, no = (int)No;
+  const int x0 = 16 * bx, y0 = 16 * by, z0 = 16 * bz;
+  auto at = [](int x, int y, int z) { return x + 17 * (y + 17 * z); };
+  double *o = out + slot * (int64_t)(512 * nbk * nbk * nbk);
+  for (int set = 0; set < (B ? 2 : 1); set++) {
+    const GT *S = set == 0 ? A : B;
+    const GT *Sw = S + slot * W3g, *Sv = S + (slots + slot) * W3g,
+             *Su = S + (2 * slots + slot) * W3g;
+    double v[16], t[16];
+    // class w: (x + No y) No + z, z fastest
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+      const int e = tid + 256 * it;
+      const int z = z0 + (e & 15), y = y0 + ((e >> 4) & 15), x = x0 + (e >> 8);
+      v[it] = (x < no && y < no && z < no) ? (double)Sw[(x + no * y) * No + z] : 0.0;
+    }
+    // class v: (x + No z) No + y, y fastest
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+      const int e = tid + 256 * it;
+      const int y = y0 + (e & 15), x = x0 + ((e >> 4) & 15), z = z0 + (e >> 8);
+      t[it] = (x < no && y < no && z < no) ? (double)Sv[(x + no * z) * No + y] : 0.0;
+    }
+    __syncthreads();  // the previous set's cube writes are done with br
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+      const int e = tid + 256 * it;
+      br[at(e >> 8, (e >> 4) & 15, e & 15)] = v[it];
+    }
+    // class u: (y + No z) No + x, x fastest
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+      const int e = tid + 256 * it;
+      const int x = x0 + (e & 15), y = y0 + ((e >> 4) & 15), z = z0 + (e >> 8);
+      v[it] = (x < no && y < no && z < no) ? (double)Su[(y + no * z) * No + x] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+      const int e = tid + 256 * it;
+      br[at((e >> 4) & 15, e & 15, e >> 8)] += t[it];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+      const int e = tid + 256 * it;
+      br[at(e & 15, (e >> 4) & 15, e >> 8)] += v[it];
+    }
+    __syncthreads();
+    // the brick's cubes, 512 contiguous doubles each; the second set adds
+    // to what the same thread wrote for the first
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+      const int e = tid + 256 * it;
+      const int cube = e >> 9, q = e & 511;
+      const int cx = cube & 1, cy = (cube >> 1) & 1, cz = cube >> 2;
+      const int gx = 2 * bx + cx, gy = 2 * by + cy, gz = 2 * bz + cz;
+      if (gx >= nbk || gy >= nbk || gz >= nbk) continue;
+      const double val = br[at(8 * cx + (q & 7), 8 * cy + ((q >> 3) & 7), 8 * cz + (q >> 6))];
+      double *dst = o + ((int64_t)((gz * nbk + gy) * nbk + gx) << 9) + q;
+      *dst = set == 0 ? val : *dst + val;
+    }
+  }
+}
+
 // Energy of one tuple per workgroup.  The i>=j>=k points are visited per
-// block triple (B1>=B2>=B3) of edge 8: the (up to) six cubes that hold W at
-// the six index permutations of those points are summed from the three class
-// buffers into LDS with coalesced 4 KiB reads, then every point reads its
-// twelve W/Z values from LDS.
-template <bool HJ, typename GT>
+// block triple (B1>=B2>=B3) of edge 8: the (up to) six distinct cubes that
+// hold W at the six index permutations of those points are copied into LDS
+// with coalesced 4 KiB reads, then every point reads its twelve W/Z values
+// from LDS.
+template <bool HJ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void energy_kernel(EnergyParams P) {
   const int64_t gslot = blockIdx.x;
@@ -76,13 +159,8 @@ void energy_kernel(EnergyParams P) {
   const double *Vbc = P.VHH + (b + Nv * c) * N2;
   const double *Vac = P.VHH + (a + Nv * c) * N2;
   const double *Vab = P.VHH + (a + Nv * b) * N2;
-  const GT *G = (const GT *)P.G, *GJ = (const GT *)P.GJ;
-  const GT *Gw = G + (0 * P.slots + gslot) * P.W3;
-  const GT *Gv = G + (1 * P.slots + gslot) * P.W3;
-  const GT *Gu = G + (2 * P.slots + gslot) * P.W3;
-  const GT *Jw = have_j ? GJ + (0 * P.slots + gslot) * P.W3 : Gw;
-  const GT *Jv = have_j ? GJ + (1 * P.slots + gslot) * P.W3 : Gv;
-  const GT *Ju = have_j ? GJ + (2 * P.slots + gslot) * P.W3 : Gu;
+  const double *Ws = P.W + gslot * P.W3;
+  const double *Js = have_j ? P.WJ + gslot * P.W3 : Ws;
 
   int distinct = 0;
   if (a == b) distinct++;
@@ -150,70 +228,46 @@ void energy_kernel(EnergyParams P) {
           }
         }
         // distinct cubes of this orbit, loaded as one flat coalesced sweep
-        int cxs[6], cys[6], czs[6];
+        int cbase[6];
         int nc = 0, slotof[6];
 #pragma unroll
         for (int p = 0; p < 6; p++) {
           slotof[p] = p;
           if (alias[p] == p) {
-            cxs[nc] = cx[p] * 8;
-            cys[nc] = cy[p] * 8;
-            czs[nc] = cz[p] * 8;
+            cbase[nc] = ((cz[p] * nbk + cy[p]) * nbk + cx[p]) << 9;
             slotof[p] = nc++;
           }
         }
 #pragma unroll
         for (int p = 0; p < 6; p++) alias[p] = slotof[alias[p]];
-        // The distinct cubes are summed (w + v) + u in LDS in three phases.
-        // Each class is read with the thread order that follows its
-        // fastest index (class w: z, v: y, u: x), so every 8 threads read
-        // 8 consecutive elements; all loads of a thread are issued before
-        // the first is used (2 elements per cube per class).
-        auto stage = [&](const GT *Cw, const GT *Cv, const GT *Cu,
-                         double (*dst)[512]) {
-          GT vw[12], vv[12], vu[12];
+        // every load of this thread is issued before the first is used
+        // (2 elements per cube per thread), so the cube fetch costs one
+        // memory latency instead of one per element
+        {
+          double vw[12];
 #pragma unroll
           for (int it = 0; it < 12; it++) {
             const int p = it >> 1, q = tid + 256 * (it & 1);
-            const int l0 = q & 7, l1 = (q >> 3) & 7, l2 = q >> 6;
-            vw[it] = vv[it] = vu[it] = (GT)0;
-            if (p < nc) {
-              {  // w: (x + No y) No + z, z fastest: (z, y, x) = (l0, l1, l2)
-                const int x = cxs[p] + l2, y = cys[p] + l1, z = czs[p] + l0;
-                if (x < no && y < no && z < no) vw[it] = Cw[(x + no * y) * No + z];
-              }
-              {  // v: (x + No z) No + y, y fastest: (y, x, z) = (l0, l1, l2)
-                const int x = cxs[p] + l1, y = cys[p] + l0, z = czs[p] + l2;
-                if (x < no && y < no && z < no) vv[it] = Cv[(x + no * z) * No + y];
-              }
-              {  // u: (y + No z) No + x, x fastest: (x, y, z) = (l0, l1, l2)
-                const int x = cxs[p] + l0, y = cys[p] + l1, z = czs[p] + l2;
-                if (x < no && y < no && z < no) vu[it] = Cu[(y + no * z) * No + x];
-              }
+            if (p < nc) vw[it] = Ws[cbase[p] + q];
+          }
+#pragma unroll
+          for (int it = 0; it < 12; it++) {
+            const int p = it >> 1, q = tid + 256 * (it & 1);
+            if (p < nc) cube[p][q] = vw[it];
+          }
+          if constexpr (HJ) {
+#pragma unroll
+            for (int it = 0; it < 12; it++) {
+              const int p = it >> 1, q = tid + 256 * (it & 1);
+              if (p < nc) vw[it] = Js[cbase[p] + q];
+            }
+#pragma unroll
+            for (int it = 0; it < 12; it++) {
+              const int p = it >> 1, q = tid + 256 * (it & 1);
+              if (p < nc) cubeJ[p][q] = vw[it];
             }
           }
-#pragma unroll
-          for (int it = 0; it < 12; it++) {
-            const int p = it >> 1, q = tid + 256 * (it & 1);
-            const int l0 = q & 7, l1 = (q >> 3) & 7, l2 = q >> 6;
-            if (p < nc) dst[p][l2 + 8 * l1 + 64 * l0] = (double)vw[it];
-          }
-          __syncthreads();
-#pragma unroll
-          for (int it = 0; it < 12; it++) {
-            const int p = it >> 1, q = tid + 256 * (it & 1);
-            const int l0 = q & 7, l1 = (q >> 3) & 7, l2 = q >> 6;
-            if (p < nc) dst[p][l1 + 8 * l0 + 64 * l2] += (double)vv[it];
-          }
-          __syncthreads();
-#pragma unroll
-          for (int it = 0; it < 12; it++) {
-            const int p = it >> 1, q = tid + 256 * (it & 1);
-            if (p < nc) dst[p][q] += (double)vu[it];
-          }
-        };
-        stage(Gw, Gv, Gu, cube);
-        if constexpr (HJ) stage(Jw, Jv, Ju, cubeJ);
+        }
         __syncthreads();
 #pragma unroll 1
         for (int pt = tid; pt < 512; pt += 256) {
@@ -350,12 +404,36 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(
 
 }  // namespace
 
+hipError_t launch_wsum(const DevSystem &s, const BatchArgs &b, const void *A,
+                       const void *B, bool g32, double *out, hipStream_t st) {
+  const int nbk = (int)w_blocks(s.No), nbr = (nbk + 1) / 2;
+  const int64_t grid = b.slots * (int64_t)nbr * nbr * nbr;
+  if (grid <= 0) return hipSuccess;
+  const int64_t W3g = s.No * s.No * s.No;
+  if (g32)
+    hipLaunchKernelGGL(wsum_kernel<float>, dim3((unsigned)grid), dim3(256), 0,
+                       st, (const float *)A, (const float *)B, out, s.No, W3g,
+                       b.slots, nbk, nbr);
+  else
+    hipLaunchKernelGGL(wsum_kernel<double>, dim3((unsigned)grid), dim3(256),
+                       0, st, (const double *)A, (const double *)B, out, s.No,
+                       W3g, b.slots, nbk, nbr);
+  return hipGetLastError();
+}
+
 hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
                          const void *G, const void *GJ, bool g32,
-                         double tai_sign, double *e_slot, hipStream_t st) {
+                         double tai_sign, double *e_slot, double *wcube,
+                         hipStream_t st) {
+  const int64_t W3c = w_cube(s.No);
+  double *W = wcube, *WJ = GJ ? wcube + b.slots * W3c : nullptr;
+  hipError_t e = launch_wsum(s, b, G, nullptr, g32, W, st);
+  if (e != hipSuccess) return e;
+  if (GJ && (e = launch_wsum(s, b, GJ, nullptr, g32, WJ, st)) != hipSuccess)
+    return e;
   EnergyParams P;
-  P.G = G;
-  P.GJ = GJ;
+  P.W = W;
+  P.WJ = WJ;
   P.Tai = s.Tai;
   P.VHH = s.VHH;
   P.ei = s.ei;
@@ -367,21 +445,17 @@ hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
   P.Nv = s.Nv;
   P.N2 = s.No * s.No;
   P.nbk = w_blocks(s.No);
-  P.W3 = s.No * s.No * s.No;
+  P.W3 = W3c;
   P.slots = b.slots;
   P.first = b.first;
   P.last = b.last;
   P.nb = b.nb;
   P.tai_sign = tai_sign;
   const dim3 grid((unsigned)b.slots), block(256);
-  if (GJ && g32)
-    hipLaunchKernelGGL((energy_kernel<true, float>), grid, block, 0, st, P);
-  else if (GJ)
-    hipLaunchKernelGGL((energy_kernel<true, double>), grid, block, 0, st, P);
-  else if (g32)
-    hipLaunchKernelGGL((energy_kernel<false, float>), grid, block, 0, st, P);
+  if (GJ)
+    hipLaunchKernelGGL(energy_kernel<true>, grid, block, 0, st, P);
   else
-    hipLaunchKernelGGL((energy_kernel<false, double>), grid, block, 0, st, P);
+    hipLaunchKernelGGL(energy_kernel<false>, grid, block, 0, st, P);
   return hipGetLastError();
 }
 

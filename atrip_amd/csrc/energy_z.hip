@@ -2,8 +2,8 @@
 //
 // The doubles of a complex system run as four real class-GEMM passes on
 // real/imaginary operand planes (see the run loop), so W = Wr + i Wi arrives
-// as four sets of class buffers (GEMM output order, see internal.hpp):
-// Wr = G0 + G1, Wi = G2 + G3, each the sum of its three classes w, v, u.  Per tuple (one workgroup), for each block
+// as four sets of class buffers: Wr = sum(G0) + sum(G1), Wi = sum(G2) +
+// sum(G3), summed into cubes by launch_wsum.  Per tuple (one workgroup), for each block
 // triple of edge 8 the cubes holding W at the six index permutations are
 // summed into LDS as (re, im), Z = W + singles is formed with complex
 // products (reference src/atrip/Equations.cxx:387-426), and the energy
@@ -38,12 +38,12 @@ __device__ inline double re_div(cplx a, cplx b) {
 }
 
 struct EnergyZParams {
-  const double *G[4], *GJ[4];  // class buffer sets: Wr = G0 + G1, Wi = G2 + G3
+  const double *Wr, *Wi, *WJr, *WJi;  // cubes, [slot][W3]
   const double *Tair, *Taii, *VHHr, *VHHi, *eir, *eii, *ear, *eai;
   const int32_t *windows;
   const uint64_t *aprefix;
   double *e_slot;
-  int64_t No, Nv, N2, W3, nbk, slots;  // nbk: 8-blocks per occupied index
+  int64_t No, Nv, N2, W3, nbk, slots;  // nbk: cubes per index, W3 = (8 nbk)^3
   uint64_t first, last;
   int nb;
   double tai_sign;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void energy_z_kernel(EnergyZParams P) {
   auto vhh = [&](int64_t base, int p, int q) {
     return cplx{P.VHHr[base + p + no * q], P.VHHi[base + p + no * q]};
   };
-  const int64_t off = gslot * P.W3, cls = P.slots * P.W3;
+  const int64_t off = gslot * P.W3;
 
   double e_acc = 0.0, ect_acc = 0.0;
 #pragma unroll 1
@@ -110,31 +110,13 @@ __global__ __launch_bounds__(256) void energy_z_kernel(EnergyZParams P) {
         const int cz[6] = {B3, B2, B3, B1, B2, B1};
         __syncthreads();
         for (int p = 0; p < 6; p++) {
+          const int64_t base = off + (((int64_t)(cz[p] * nbk + cy[p]) * nbk + cx[p]) << 9);
           for (int q = tid; q < 512; q += 256) {
-            const int x = cx[p] * 8 + (q & 7), y = cy[p] * 8 + ((q >> 3) & 7),
-                      z = cz[p] * 8 + (q >> 6);
-            double r0 = 0, r1 = 0, i0 = 0, i1 = 0, s0 = 0, s1 = 0, t0 = 0, t1 = 0;
-            if (x < no && y < no && z < no) {
-              // class buffers in GEMM order: w, v, u
-              const int64_t ew = off + (x + No * y) * No + z;
-              const int64_t ev = off + cls + (x + No * z) * No + y;
-              const int64_t eu = off + 2 * cls + (y + No * z) * No + x;
-              r0 = (P.G[0][ew] + P.G[0][ev]) + P.G[0][eu];
-              r1 = (P.G[1][ew] + P.G[1][ev]) + P.G[1][eu];
-              i0 = (P.G[2][ew] + P.G[2][ev]) + P.G[2][eu];
-              i1 = (P.G[3][ew] + P.G[3][ev]) + P.G[3][eu];
-              if constexpr (HJ) {
-                s0 = (P.GJ[0][ew] + P.GJ[0][ev]) + P.GJ[0][eu];
-                s1 = (P.GJ[1][ew] + P.GJ[1][ev]) + P.GJ[1][eu];
-                t0 = (P.GJ[2][ew] + P.GJ[2][ev]) + P.GJ[2][eu];
-                t1 = (P.GJ[3][ew] + P.GJ[3][ev]) + P.GJ[3][eu];
-              }
-            }
-            cr[p][q] = r0 + r1;
-            ci[p][q] = i0 + i1;
+            cr[p][q] = P.Wr[base + q];
+            ci[p][q] = P.Wi[base + q];
             if constexpr (HJ) {
-              jr[p][q] = s0 + s1;
-              ji[p][q] = t0 + t1;
+              jr[p][q] = P.WJr[base + q];
+              ji[p][q] = P.WJi[base + q];
             }
           }
         }
@@ -209,12 +191,22 @@ __global__ __launch_bounds__(256) void energy_z_kernel(EnergyZParams P) {
 
 hipError_t launch_energy_z(const DevSystem &s, const BatchArgs &b,
                            double *const G[4], double *const GJ[4],
-                           double tai_sign, double *e_slot, hipStream_t st) {
+                           double tai_sign, double *e_slot, double *wcube,
+                           hipStream_t st) {
+  const int64_t W3c = w_cube(s.No), n = b.slots * W3c;
   EnergyZParams P;
-  for (int q = 0; q < 4; q++) {
-    P.G[q] = G[q];
-    P.GJ[q] = GJ ? GJ[q] : nullptr;
-  }
+  P.Wr = wcube;
+  P.Wi = wcube + n;
+  P.WJr = GJ ? wcube + 2 * n : nullptr;
+  P.WJi = GJ ? wcube + 3 * n : nullptr;
+  hipError_t e;
+  if ((e = launch_wsum(s, b, G[0], G[1], false, (double *)P.Wr, st)) != hipSuccess ||
+      (e = launch_wsum(s, b, G[2], G[3], false, (double *)P.Wi, st)) != hipSuccess)
+    return e;
+  if (GJ &&
+      ((e = launch_wsum(s, b, GJ[0], GJ[1], false, (double *)P.WJr, st)) != hipSuccess ||
+       (e = launch_wsum(s, b, GJ[2], GJ[3], false, (double *)P.WJi, st)) != hipSuccess))
+    return e;
   P.Tair = s.Tai;
   P.Taii = s.Taii;
   P.VHHr = s.VHH;
@@ -230,7 +222,7 @@ hipError_t launch_energy_z(const DevSystem &s, const BatchArgs &b,
   P.Nv = s.Nv;
   P.N2 = s.No * s.No;
   P.nbk = w_blocks(s.No);
-  P.W3 = s.No * s.No * s.No;
+  P.W3 = W3c;
   P.slots = b.slots;
   P.first = b.first;
   P.last = b.last;

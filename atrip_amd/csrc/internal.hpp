@@ -30,6 +30,13 @@ constexpr int WCUBE = 8;
 __host__ __device__ inline int64_t w_blocks(int64_t No) {
   return (No + WCUBE - 1) / WCUBE;
 }
+// Doubles of one tuple's W in the cube layout the energy kernels read: the
+// occupied range padded to whole cubes, cube (bx, by, bz) at
+// ((bz nbk + by) nbk + bx) * 512, element x + 8 y + 64 z inside.
+__host__ __device__ inline int64_t w_cube(int64_t No) {
+  const int64_t n = w_blocks(No) * WCUBE;
+  return n * n * n;
+}
 
 // Device-resident system.  Pair-indexed arrays use the CTF pair order
 // p(x,y) = x + Nv*y.  Layouts (all row-major with the last index fastest):
@@ -137,6 +144,8 @@ struct atrip_hip_ctx {
   // scratch, grown on demand
   double *G = nullptr;       // class buffers [ncls][slots][No^3]
   size_t G_bytes = 0;
+  double *wcube = nullptr;   // W as cubes for the energy kernels
+  size_t wcube_bytes = 0;
   double *e_slot = nullptr;  // per-slot energies (2 per slot: e, e_ct)
   size_t e_slot_bytes = 0;
   double *partials = nullptr;  // per-batch (sum, comp) x 2
@@ -182,16 +191,25 @@ hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
                                 bool use_j, float *G, bool corrections,
                                 int tile, hipStream_t st);
 
-// Singles + energy epilogue; writes e_slot[2*slot + {0,1}].
+// Class sums: out[slot] = sum(A) (+ sum(B)) as cubes (w_cube per slot), for
+// class-buffer sets A, B (FP64, or FP32 when g32).
+hipError_t launch_wsum(const DevSystem &s, const BatchArgs &b, const void *A,
+                       const void *B, bool g32, double *out, hipStream_t st);
+
+// Class sums and singles + energy epilogue; writes e_slot[2*slot + {0,1}].
+// wcube: scratch of (GJ ? 2 : 1) * slots * w_cube(No) doubles.
 hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
                          const void *G, const void *GJ, bool g32,
-                         double tai_sign, double *e_slot, hipStream_t st);
+                         double tai_sign, double *e_slot, double *wcube,
+                         hipStream_t st);
 
 // Singles + energy of complex systems from the four class-buffer sets of
 // the real passes (Wr = G0 + G1, Wi = G2 + G3); GJ null without (cT).
+// wcube: scratch of (GJ ? 4 : 2) * slots * w_cube(No) doubles.
 hipError_t launch_energy_z(const DevSystem &s, const BatchArgs &b,
                            double *const G[4], double *const GJ[4],
-                           double tai_sign, double *e_slot, hipStream_t st);
+                           double tai_sign, double *e_slot, double *wcube,
+                           hipStream_t st);
 
 // Deterministic compensated sum of the valid slot energies of a batch into
 // partials[4*index + {sum, comp, sum_ct, comp_ct}].
