@@ -60,45 +60,51 @@ __global__ __launch_bounds__(256) void wsum_kernel(const GT *A, const GT *B,
     const GT *S = set == 0 ? A : B;
     const GT *Sw = S + slot * W3g, *Sv = S + (slots + slot) * W3g,
              *Su = S + (2 * slots + slot) * W3g;
-    double v[16], t[16];
+    GT v[16], t[16];  // widened to FP64 only when added into the brick
     // class w: (x + No y) No + z, z fastest
 #pragma unroll
     for (int it = 0; it < 16; it++) {
       const int e = tid + 256 * it;
       const int z = z0 + (e & 15), y = y0 + ((e >> 4) & 15), x = x0 + (e >> 8);
-      v[it] = (x < no && y < no && z < no) ? (double)Sw[(x + no * y) * No + z] : 0.0;
+      const bool in = x < no && y < no && z < no;
+      v[it] = Sw[in ? (x + no * y) * No + z : 0];  // always a valid address
+      if (!in) v[it] = (GT)0;
     }
     // class v: (x + No z) No + y, y fastest
 #pragma unroll
     for (int it = 0; it < 16; it++) {
       const int e = tid + 256 * it;
       const int y = y0 + (e & 15), x = x0 + ((e >> 4) & 15), z = z0 + (e >> 8);
-      t[it] = (x < no && y < no && z < no) ? (double)Sv[(x + no * z) * No + y] : 0.0;
+      const bool in = x < no && y < no && z < no;
+      t[it] = Sv[in ? (x + no * z) * No + y : 0];
+      if (!in) t[it] = (GT)0;
     }
     __syncthreads();  // the previous set's cube writes are done with br
 #pragma unroll
     for (int it = 0; it < 16; it++) {
       const int e = tid + 256 * it;
-      br[at(e >> 8, (e >> 4) & 15, e & 15)] = v[it];
+      br[at(e >> 8, (e >> 4) & 15, e & 15)] = (double)v[it];
     }
     // class u: (y + No z) No + x, x fastest
 #pragma unroll
     for (int it = 0; it < 16; it++) {
       const int e = tid + 256 * it;
       const int x = x0 + (e & 15), y = y0 + ((e >> 4) & 15), z = z0 + (e >> 8);
-      v[it] = (x < no && y < no && z < no) ? (double)Su[(y + no * z) * No + x] : 0.0;
+      const bool in = x < no && y < no && z < no;
+      v[it] = Su[in ? (y + no * z) * No + x : 0];
+      if (!in) v[it] = (GT)0;
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < 16; it++) {
       const int e = tid + 256 * it;
-      br[at((e >> 4) & 15, e & 15, e >> 8)] += t[it];
+      br[at((e >> 4) & 15, e & 15, e >> 8)] += (double)t[it];
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < 16; it++) {
       const int e = tid + 256 * it;
-      br[at(e & 15, (e >> 4) & 15, e >> 8)] += v[it];
+      br[at(e & 15, (e >> 4) & 15, e >> 8)] += (double)v[it];
     }
     __syncthreads();
     // the brick's cubes, 512 contiguous doubles each; the second set adds
@@ -245,10 +251,12 @@ void energy_kernel(EnergyParams P) {
         // memory latency instead of one per element
         {
           double vw[12];
+          // loads are unconditional (a valid cube when p >= nc) so none
+          // waits behind a branch
 #pragma unroll
           for (int it = 0; it < 12; it++) {
             const int p = it >> 1, q = tid + 256 * (it & 1);
-            if (p < nc) vw[it] = Ws[cbase[p] + q];
+            vw[it] = Ws[cbase[p < nc ? p : 0] + q];
           }
 #pragma unroll
           for (int it = 0; it < 12; it++) {
@@ -259,7 +267,7 @@ void energy_kernel(EnergyParams P) {
 #pragma unroll
             for (int it = 0; it < 12; it++) {
               const int p = it >> 1, q = tid + 256 * (it & 1);
-              if (p < nc) vw[it] = Js[cbase[p] + q];
+              vw[it] = Js[cbase[p < nc ? p : 0] + q];
             }
 #pragma unroll
             for (int it = 0; it < 12; it++) {
