@@ -32,7 +32,6 @@ void atrip_amd::ensure_split(atrip_hip_ctx *ctx, DevSystem &s) {
   auto make = [&](const double *x, DevSystem::Split &o, int64_t n) {
     if (!x) return;
     o.h = (float *)dev_bytes(s, (size_t)n * sizeof(float), "alloc split");
-    o.b = (uint16_t *)dev_bytes(s, (size_t)n * 2, "alloc split");
     o.l = (uint16_t *)dev_bytes(s, (size_t)n * 2, "alloc split");
     check(launch_split(x, o, n, ctx->stream), "split conversion");
   };
@@ -80,10 +79,10 @@ uint64_t atrip_amd::system_bytes(int64_t No, int64_t Nv, bool with_j,
                                  int store) {
   const uint64_t Kv = pad16(Nv), Ko = pad16(No), N2 = No * No, Nv2 = Nv * Nv;
   const auto d = [](uint64_t n) { return std::max<uint64_t>(n, 1) * 8; };
-  // doubles operands: 8 bytes per element as FP64 or as split (4 + 2 + 2),
+  // doubles operands: 8 bytes per element as FP64, 6 as split (4 + 2),
   // 4 as FP32 only
   const auto o = [&](uint64_t n) {
-    return store == 2 ? n * 4 : store == 1 ? n * 8 : d(n);
+    return store == 2 ? n * 4 : store == 1 ? n * 6 : d(n);
   };
   uint64_t b = o(Nv * N2 * Kv) + o(Nv2 * No * Kv) + o(Nv * N2 * Ko) +
                2 * o(Nv2 * No * Ko) + d(Nv2 * N2) + d(Nv * No) + d(No) + d(Nv);
@@ -116,7 +115,6 @@ void alloc_system(atrip_hip_ctx *ctx, DevSystem &s, int64_t No, int64_t Nv,
     }
     sp.h = (float *)dev_bytes(s, (size_t)n * sizeof(float), what);
     if (s.store == 1) {
-      sp.b = (uint16_t *)dev_bytes(s, (size_t)n * 2, what);
       sp.l = (uint16_t *)dev_bytes(s, (size_t)n * 2, what);
     }
   };

@@ -170,8 +170,6 @@ BSplit make_bsplit(const DevSystem &s) {
   BSplit x;
   x.T3h = s.sT3.h; x.V3h = s.sV3.h; x.H3h = s.sH3.h;
   x.T2Nh = s.sT2N.h; x.T2TNh = s.sT2TN.h;
-  x.T3b = s.sT3.b; x.V3b = s.sV3.b; x.H3b = s.sH3.b;
-  x.T2Nb = s.sT2N.b; x.T2TNb = s.sT2TN.b;
   x.T3l = s.sT3.l; x.V3l = s.sV3.l; x.H3l = s.sH3.l;
   x.T2Nl = s.sT2N.l; x.T2TNl = s.sT2TN.l;
   return x;

@@ -28,7 +28,19 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// correction-MFMA operands from four hi and four lo values of one kappa
+// range: A [bf16(hi), lo], B [lo, bf16(hi)] (as the plain split kernel)
+__device__ inline bf8 corr_a(f4 h, bf4 l) {
+  const bf4 b = __builtin_convertvector(h, bf4);
+  return __builtin_shufflevector(b, l, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+__device__ inline bf8 corr_b(f4 h, bf4 l) {
+  const bf4 b = __builtin_convertvector(h, bf4);
+  return __builtin_shufflevector(l, b, 0, 1, 2, 3, 4, 5, 6, 7);
+}
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -166,9 +178,9 @@ __device__ __forceinline__ void wave_tile(const BSys &S, int cl, int al,
 
 // The same wave tile with FP32 products (mixed forms): hi*hi on
 // v_mfma_f32_16x16x4f32 and, when CORR, hi*lo + lo*hi in one
-// v_mfma_f32_16x16x32_bf16 per 16-deep chunk (lanes 0..31 carry hi_a and
-// lo_b, lanes 32..63 lo_a and hi_b), FP32 accumulation; the tile is widened
-// to FP64 when it is added into W.
+// v_mfma_f32_16x16x32_bf16 per 16-deep chunk (each lane group pairs
+// [bf16(hi_a), lo_a] with [lo_b, bf16(hi_b)] over its four kappa), FP32
+// accumulation; the tile is widened to FP64 when it is added into W.
 template <int FM, int FN, bool CORR>
 __device__ __forceinline__ void wave_tile_split(const BSys &S, const BSplit &X,
                                                 int cl, int al, int be, int ga,
@@ -209,9 +221,6 @@ __device__ __forceinline__ void wave_tile_split(const BSys &S, const BSplit &X,
     rowS[f] = (int32_t)m;
     rowW[f] = (int32_t)(m / No + No * (m % No));
   }
-  const bool lower = kk < 2;
-  const int coff = 8 * (kk & 1);
-
   f4 acc[FM][FN];
 #pragma unroll
   for (int f = 0; f < FM; f++)
@@ -220,16 +229,16 @@ __device__ __forceinline__ void wave_tile_split(const BSys &S, const BSplit &X,
   const float *aph[FM], *bph[FN];
   const __bf16 *apc[FM], *bpc[FN];
   f4 ah0[FM], bh0[FN], ah1[FM], bh1[FN];
-  bf8 ac0[FM], bc0[FN], ac1[FM], bc1[FN];
+  bf4 ac0[FM], bc0[FN], ac1[FM], bc1[FN];
 #define LOAD(AH, AC, BH, BC, KOFF)                                          \
   {                                                                         \
     _Pragma("unroll") for (int f = 0; f < FM; f++) {                        \
       AH[f] = *(const f4 *)(aph[f] + (KOFF));                               \
-      if constexpr (CORR) AC[f] = *(const bf8 *)(apc[f] + (KOFF));         \
+      if constexpr (CORR) AC[f] = *(const bf4 *)(apc[f] + (KOFF));         \
     }                                                                       \
     _Pragma("unroll") for (int g = 0; g < FN; g++) {                        \
       BH[g] = *(const f4 *)(bph[g] + (KOFF));                               \
-      if constexpr (CORR) BC[g] = *(const bf8 *)(bpc[g] + (KOFF));         \
+      if constexpr (CORR) BC[g] = *(const bf4 *)(bpc[g] + (KOFF));         \
     }                                                                       \
     __builtin_amdgcn_sched_barrier(0);                                      \
   }
@@ -240,11 +249,17 @@ __device__ __forceinline__ void wave_tile_split(const BSys &S, const BSplit &X,
             _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =      \
                 __builtin_amdgcn_mfma_f32_16x16x4f32(AH[f][s], BH[g][s],    \
                                                      acc[f][g], 0, 0, 0);   \
-    if constexpr (CORR)                                                     \
+    if constexpr (CORR) {                                                   \
+      bf8 ca_[FM], cb_[FN];                                                 \
+      _Pragma("unroll") for (int f = 0; f < FM; f++) ca_[f] =               \
+          corr_a(AH[f], AC[f]);                                             \
+      _Pragma("unroll") for (int g = 0; g < FN; g++) cb_[g] =               \
+          corr_b(BH[g], BC[g]);                                             \
       _Pragma("unroll") for (int f = 0; f < FM; f++)                        \
           _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =        \
-              __builtin_amdgcn_mfma_f32_16x16x32_bf16(AC[f], BC[g],         \
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(ca_[f], cb_[g],       \
                                                       acc[f][g], 0, 0, 0);  \
+    }                                                                       \
   }
 #pragma unroll 1
   for (int sg = 0; sg < 4; sg++) {
@@ -252,15 +267,12 @@ __device__ __forceinline__ void wave_tile_split(const BSys &S, const BSplit &X,
     const int64_t xa = (sg == 0 || sg == 3) ? x1 : x2;
     const int64_t abase = xa * N2 * K_;
     const float *Ah = (sg < 2 ? X.T3h : X.H3h) + abase;
-    const __bf16 *Ac =
-        (const __bf16 *)(lower ? (sg < 2 ? X.T3b : X.H3b)
-                               : (sg < 2 ? X.T3l : X.H3l)) +
-        abase;
+    const __bf16 *Ac = (const __bf16 *)(sg < 2 ? X.T3l : X.H3l) + abase;
 #pragma unroll
     for (int f = 0; f < FM; f++) {
       const int64_t r = (int64_t)((sg & 1) ? rowW[f] : rowS[f]) * K_;
       aph[f] = Ah + r + 4 * kk;
-      apc[f] = Ac + r + coff;
+      apc[f] = Ac + r + 4 * kk;
     }
 #pragma unroll
     for (int g = 0; g < FN; g++) {
@@ -268,10 +280,9 @@ __device__ __forceinline__ void wave_tile_split(const BSys &S, const BSplit &X,
       const int wh = b_pair(cl, sg, ca[g], cb[g], cc[g], u, v);
       const int64_t c = ((u + Nv * v) * No + crr[g]) * K_;
       const float *Bh = wh == 0 ? X.V3h : wh == 1 ? X.T2Nh : X.T2TNh;
-      const uint16_t *Bc = lower ? (wh == 0 ? X.V3l : wh == 1 ? X.T2Nl : X.T2TNl)
-                                 : (wh == 0 ? X.V3b : wh == 1 ? X.T2Nb : X.T2TNb);
+      const uint16_t *Bc = wh == 0 ? X.V3l : wh == 1 ? X.T2Nl : X.T2TNl;
       bph[g] = Bh + c + 4 * kk;
-      bpc[g] = (const __bf16 *)Bc + c + coff;
+      bpc[g] = (const __bf16 *)Bc + c + 4 * kk;
     }
     const int64_t last = K_ - KCHUNK;
     LOAD(ah0, ac0, bh0, bc0, 0);

@@ -2,15 +2,19 @@
 //
 // Same regrouped class GEMMs and tuple windows as the FP64 kernel
 // (doubles_f64.hip).  Every operand x is held as hi = (float)x plus the
-// residual lo = x - hi; a product is formed as
+// residual lo = bf16(x - hi); a product is formed as
 //     hi_a hi_b                      on v_mfma_f32_16x16x4_f32
 //   + (hi_a lo_b + lo_a hi_b)        on one v_mfma_f32_16x16x32_bf16
 // (the FP32 form drops the bracket: one FP32 product per product)
-// with both MFMAs accumulating into the same FP32 accumulator: for a K chunk
-// of 16 the bf16 instruction's K = 32 holds the 16 hi*lo products (lanes
-// 0..31) and the 16 lo*hi products (lanes 32..63).  The correction terms
-// carry the part of each product below FP32's 24 bits (their own bf16
-// rounding is 2^-8 of an already 2^-24-small term).  The FP32 result of each
+// with both MFMAs accumulating into the same FP32 accumulator.  For a K chunk
+// of 16, lane group q (lanes 16q..16q+15) holds kappa = k0 + 4q + s, s < 4,
+// of its row/column for the FP32 MFMAs; the bf16 instruction's 8 K slots of
+// the same lane group take [bf16(hi_a), lo_a] against [lo_b, bf16(hi_b)] at
+// those four kappa, so its K = 32 covers the 16 hi*lo and the 16 lo*hi
+// products of the chunk and bf16(hi) is made in registers from the hi
+// already loaded (only hi and lo are stored: 6 bytes per element).  The
+// correction terms carry the part of each product below FP32's 24 bits
+// (their own bf16 rounding is 2^-8 of an already 2^-24-small term).  The FP32 result of each
 // class GEMM is stored as FP32 (it carries no more bits) and widened to FP64
 // by the energy kernel; singles, the energy and all sums stay FP64
 // (energy.hip).
@@ -20,12 +24,23 @@ namespace atrip_amd {
 namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
+// bf16 operands of the correction MFMA from a lane's four hi values h and
+// four lo values l (same kappa): A takes [bf16(h), l], B takes [l, bf16(h)].
+__device__ inline bf8 corr_a(f4 h, bf4 l) {
+  const bf4 b = __builtin_convertvector(h, bf4);
+  return __builtin_shufflevector(b, l, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+__device__ inline bf8 corr_b(f4 h, bf4 l) {
+  const bf4 b = __builtin_convertvector(h, bf4);
+  return __builtin_shufflevector(l, b, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
 struct SplitParams {
-  // per operand family: hi (float), hi as bf16, lo as bf16
+  // per operand family: hi (float) and lo (bf16)
   const float *T3h, *H3h, *V3h, *T2Nh, *T2TNh;
-  const __bf16 *T3b, *H3b, *V3b, *T2Nb, *T2TNb;
   const __bf16 *T3l, *H3l, *V3l, *T2Nl, *T2TNl;
   const int32_t *windows;
   const uint64_t *aprefix;
@@ -96,9 +111,6 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
   }
 
   const int64_t Kv = P.Kv, Ko = P.Ko;
-  // lanes 0..31 of the bf16 MFMA take hi_a*lo_b, lanes 32..63 lo_a*hi_b
-  const bool lower = kk < 2;
-  const int64_t coff = 8 * (kk & 1);
   const float *aph[FM], *bph[FN];
   const __bf16 *apc[FM], *bpc[FN];
 #define SET_PTRS(SG)                                                          \
@@ -107,50 +119,52 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
     const int64_t xa_ = ((SG) == 0 || (SG) == 3) ? x1 : x2;                   \
     const int64_t abase_ = xa_ * N2 * K_;                                     \
     const float *Ah_ = ((SG) < 2 ? P.T3h : P.H3h) + abase_;                   \
-    const __bf16 *Ac_ =                                                       \
-        (lower ? ((SG) < 2 ? P.T3b : P.H3b) : ((SG) < 2 ? P.T3l : P.H3l)) +   \
-        abase_;                                                               \
+    const __bf16 *Al_ = ((SG) < 2 ? P.T3l : P.H3l) + abase_;                  \
     _Pragma("unroll") for (int f = 0; f < FM; f++) {                          \
       const int64_t r_ = (((SG)&1) ? rowW[f] : rowS[f]) * K_;                 \
       aph[f] = Ah_ + r_ + 4 * kk;                                             \
-      apc[f] = Ac_ + r_ + coff;                                               \
+      apc[f] = Al_ + r_ + 4 * kk;                                             \
     }                                                                         \
     _Pragma("unroll") for (int g = 0; g < FN; g++) {                          \
       int64_t u_, v_;                                                         \
       const int wh_ = b_pair(cl, (SG), ca[g], cb[g], cc[g], u_, v_);          \
       const int64_t c_ = ((u_ + Nv * v_) * No + crr[g]) * K_;                 \
       const float *Bh_ = wh_ == 0 ? P.V3h : wh_ == 1 ? P.T2Nh : P.T2TNh;      \
-      const __bf16 *Bc_ =                                                     \
-          lower ? (wh_ == 0 ? P.V3l : wh_ == 1 ? P.T2Nl : P.T2TNl)            \
-                : (wh_ == 0 ? P.V3b : wh_ == 1 ? P.T2Nb : P.T2TNb);           \
+      const __bf16 *Bl_ = wh_ == 0 ? P.V3l : wh_ == 1 ? P.T2Nl : P.T2TNl;     \
       bph[g] = Bh_ + c_ + 4 * kk;                                             \
-      bpc[g] = Bc_ + c_ + coff;                                               \
+      bpc[g] = Bl_ + c_ + 4 * kk;                                             \
     }                                                                         \
   }
-#define LOAD(AH, AC, BH, BC, KOFF)                                           \
+#define LOAD(AH, AL, BH, BL, KOFF)                                           \
   {                                                                          \
     _Pragma("unroll") for (int f = 0; f < FM; f++) {                         \
       AH[f] = *(const f4 *)(aph[f] + (KOFF));                                \
-      if constexpr (CORR) AC[f] = *(const bf8 *)(apc[f] + (KOFF));          \
+      if constexpr (CORR) AL[f] = *(const bf4 *)(apc[f] + (KOFF));          \
     }                                                                        \
     _Pragma("unroll") for (int g = 0; g < FN; g++) {                         \
       BH[g] = *(const f4 *)(bph[g] + (KOFF));                                \
-      if constexpr (CORR) BC[g] = *(const bf8 *)(bpc[g] + (KOFF));          \
+      if constexpr (CORR) BL[g] = *(const bf4 *)(bpc[g] + (KOFF));          \
     }                                                                        \
     __builtin_amdgcn_sched_barrier(0);                                       \
   }
-#define COMPUTE(AH, AC, BH, BC)                                              \
+#define COMPUTE(AH, AL, BH, BL)                                              \
   {                                                                          \
     _Pragma("unroll") for (int s = 0; s < 4; s++)                            \
         _Pragma("unroll") for (int f = 0; f < FM; f++)                       \
             _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =       \
                 __builtin_amdgcn_mfma_f32_16x16x4f32(AH[f][s], BH[g][s],     \
                                                      acc[f][g], 0, 0, 0);    \
-    if constexpr (CORR)                                                      \
+    if constexpr (CORR) {                                                    \
+      bf8 ca_[FM], cb_[FN];                                                  \
+      _Pragma("unroll") for (int f = 0; f < FM; f++) ca_[f] =                \
+          corr_a(AH[f], AL[f]);                                              \
+      _Pragma("unroll") for (int g = 0; g < FN; g++) cb_[g] =                \
+          corr_b(BH[g], BL[g]);                                              \
       _Pragma("unroll") for (int f = 0; f < FM; f++)                         \
           _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =         \
-              __builtin_amdgcn_mfma_f32_16x16x32_bf16(AC[f], BC[g],          \
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(ca_[f], cb_[g],        \
                                                       acc[f][g], 0, 0, 0);   \
+    }                                                                        \
   }
 
   f4 acc[FM][FN];
@@ -163,7 +177,7 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
   // behind the MFMAs of the current one (FP32 MFMAs take half the cycles of
   // FP64 ones, so one chunk of compute alone does not cover the latency)
   f4 ah0[FM], bh0[FN], ah1[FM], bh1[FN], ah2[FM], bh2[FN];
-  bf8 ac0[FM], bc0[FN], ac1[FM], bc1[FN], ac2[FM], bc2[FN];
+  bf4 ac0[FM], bc0[FN], ac1[FM], bc1[FN], ac2[FM], bc2[FN];
 #define CLAMP(K_) ((K_) < last ? (K_) : last)
 #pragma unroll 1
   for (int sg = 0; sg < 4; sg++) {
@@ -209,14 +223,13 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
   }
 }
 
-// x -> hi = (float)x, b = bf16(hi), l = bf16(x - hi); bf16 by round to
-// nearest even on the float bits.
+// x -> hi = (float)x, l = bf16(x - hi); bf16 by round to nearest even on
+// the float bits.
 __global__ __launch_bounds__(256) void split_kernel(const double *x, float *h,
-                                                    uint16_t *b, uint16_t *l,
-                                                    int64_t n) {
+                                                    uint16_t *l, int64_t n) {
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
-    split_store(x[q], h, b, l, q);
+    split_store(x[q], h, l, q);
   }
 }
 
@@ -228,7 +241,7 @@ hipError_t launch_split(const double *x, DevSystem::Split &out, int64_t n,
   int64_t blocks = (n + 255) / 256;
   if (blocks > 65536L * 16) blocks = 65536L * 16;
   hipLaunchKernelGGL(split_kernel, dim3((unsigned)blocks), dim3(256), 0, st, x,
-                     out.h, out.b, out.l, n);
+                     out.h, out.l, n);
   return hipGetLastError();
 }
 
@@ -240,9 +253,6 @@ hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
   const DevSystem::Split &H = use_j ? s.sH3J : s.sH3;
   P.T3h = s.sT3.h; P.H3h = H.h; P.V3h = V.h; P.T2Nh = s.sT2N.h;
   P.T2TNh = s.sT2TN.h;
-  P.T3b = (const __bf16 *)s.sT3.b; P.H3b = (const __bf16 *)H.b;
-  P.V3b = (const __bf16 *)V.b; P.T2Nb = (const __bf16 *)s.sT2N.b;
-  P.T2TNb = (const __bf16 *)s.sT2TN.b;
   P.T3l = (const __bf16 *)s.sT3.l; P.H3l = (const __bf16 *)H.l;
   P.V3l = (const __bf16 *)V.l; P.T2Nl = (const __bf16 *)s.sT2N.l;
   P.T2TNl = (const __bf16 *)s.sT2TN.l;

@@ -64,17 +64,18 @@ struct DevSystem {
          *eai = nullptr, *V3Ji = nullptr, *H3Ji = nullptr;
   double *T3in = nullptr, *H3in = nullptr, *H3Jin = nullptr;
   // Split-FP32 copies of the GEMM operands, made on the first mixed run:
-  // h = (float)x, b = bf16(h), l = bf16(x - h), same layouts as above.
+  // h = (float)x and l = bf16(x - h), same layouts as above (bf16(h), the
+  // other factor of the correction products, is formed in registers).
   struct Split {
     float *h = nullptr;
-    uint16_t *b = nullptr, *l = nullptr;
+    uint16_t *l = nullptr;
   };
   Split sT3, sV3, sH3, sT2N, sT2TN, sV3J, sH3J;
   bool split_ready = false;
   // Operand store: 0 FP64 (split copies made on the first mixed run),
-  // 1 split FP32 only, 2 FP32 (h) only.  Modes 1 and 2 keep no FP64 copy of
-  // the doubles operands (T3, V3, H3, T2N, T2TN and the J arrays), which
-  // halves (mode 2) the resident bytes of the large systems.
+  // 1 split FP32 only (h + l, 6 bytes per element), 2 FP32 (h) only.  Modes
+  // 1 and 2 keep no FP64 copy of the doubles operands (T3, V3, H3, T2N, T2TN
+  // and the J arrays).
   int store = 0;
   uint32_t *points = nullptr;             // packed (i>=j>=k) triples
   int64_t npoints = 0;
@@ -90,16 +91,13 @@ __device__ inline uint16_t to_bf16(float v) {
   return (uint16_t)(u >> 16);
 }
 
-// Split-FP32 element q of x: h = (float)x, b = bf16(h), l = bf16(x - h);
-// b and l may be null (FP32-only store).
-__device__ inline void split_store(double x, float *h, uint16_t *b,
-                                   uint16_t *l, int64_t q) {
+// Split-FP32 element q of x: h = (float)x, l = bf16(x - h); l may be null
+// (FP32-only store).
+__device__ inline void split_store(double x, float *h, uint16_t *l,
+                                   int64_t q) {
   const float hi = (float)x;
   h[q] = hi;
-  if (b) {
-    b[q] = to_bf16(hi);
-    l[q] = to_bf16((float)(x - (double)hi));
-  }
+  if (l) l[q] = to_bf16((float)(x - (double)hi));
 }
 
 // Tuple windows: block triples (alpha <= beta <= gamma) of edge nb.
@@ -257,10 +255,9 @@ constexpr __host__ __device__ inline int bfn(int v) {
 }
 
 // Split-FP32 operands of a system in a batched launch (h = FP32 values,
-// b = bf16(h), l = bf16(x - h); b and l unused by the FP32-only form).
+// l = bf16(x - h); l unused by the FP32-only form).
 struct BSplit {
   const float *T3h, *V3h, *H3h, *T2Nh, *T2TNh;
-  const uint16_t *T3b, *V3b, *H3b, *T2Nb, *T2TNb;
   const uint16_t *T3l, *V3l, *H3l, *T2Nl, *T2TNl;
 };
 

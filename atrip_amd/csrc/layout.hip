@@ -45,7 +45,7 @@ enum SynthTarget { S_T3, S_V3, S_H3, S_T2N, S_T2TN, S_VHH, S_PLAIN };
 // Fill one device array from the generator; element q of the destination is
 // mapped back to the CTF linear index of the tensor it holds.
 __global__ __launch_bounds__(256) void synth_kernel(double *dst, float *sh,
-                                                    uint16_t *sb, uint16_t *sl,
+                                                    uint16_t *sl,
                                                     int64_t n,
                                                     int target, int tensor,
                                                     uint64_t stream, int64_t No,
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void synth_kernel(double *dst, float *sh,
       v = sign * atrip_synth_value(stream, tensor, (uint64_t)No, (uint64_t)Nv,
                                    (uint64_t)idx);
     if (dst) dst[q] = v;
-    if (sh) split_store(v, sh, sb, sl, q);
+    if (sh) split_store(v, sh, sl, q);
   }
 }
 
@@ -110,7 +110,7 @@ hipError_t synth_fill(double *dst, const DevSystem::Split &o, int64_t n,
   int64_t blocks = (n + 255) / 256;
   if (blocks > 65536L * 16) blocks = 65536L * 16;
   hipLaunchKernelGGL(synth_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
-                     dst, o.h, o.b, o.l, n, (int)target, tensor, stream, s.No,
+                     dst, o.h, o.l, n, (int)target, tensor, stream, s.No,
                      s.Nv, s.Kv, s.Ko);
   return hipGetLastError();
 }
