@@ -57,6 +57,68 @@ __device__ __forceinline__ const double *bbase(const BSys &S, int cl, int sg, in
   return Bt + (u + S.Nv * v) * S.No * (sg < 2 ? S.Kv : S.Ko);
 }
 
+// Adds one wave tile into W (order i + No j + No^2 k).  Class u's column
+// index is i, so its fragment rows are stored as they are (16 consecutive
+// i per row, 128 bytes).  For classes w and v the fragment is first
+// transposed through the wave's 16 x 17 LDS area so that a store
+// instruction covers 16 consecutive rows m (consecutive i) of 4 columns.
+// OUT = 8: FP64 MFMA layout D[i][j] in lane j + 16 (i % 4), register i / 4;
+// OUT = 4: FP32 MFMA layout D[i][j] in lane j + 16 (i / 4), register i % 4.
+template <int FM, int FN, int OUT, typename V4>
+__device__ __forceinline__ void store_tile(const V4 (&acc)[FM][FN], int cl,
+                                           int mt, int nt, int64_t No,
+                                           const bool *cvalid,
+                                           const int32_t *cslot,
+                                           const int32_t *crr, double *Gwin,
+                                           bool add, int lane, double *xp) {
+  const int kk = lane >> 4, l16 = lane & 15;
+  const int no = (int)No, n2 = no * no, w3 = n2 * no;
+#pragma unroll
+  for (int g = 0; g < FN; g++) {
+#pragma unroll
+    for (int f = 0; f < FM; f++) {
+      if (cl == 2) {
+        if (!cvalid[g]) continue;
+        double *Gs = Gwin + cslot[g] * w3;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int row = OUT == 8 ? 4 * r + kk : 4 * kk + r;
+          const int m = mt * 16 * FM + f * 16 + row;
+          if (m >= n2) continue;
+          const int p = m % no, q = m / no;
+          double *dst = Gs + crr[g] + no * (p + no * q);
+          const double v = (double)acc[f][g][r];
+          *dst = add ? *dst + v : v;
+        }
+        continue;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = OUT == 8 ? 4 * r + kk : 4 * kk + r;
+        xp[row * 17 + l16] = (double)acc[f][g][r];
+      }
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): LDS writes done
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = 4 * r + kk, row = l16;
+        const double v = xp[row * 17 + col];
+        const int slot = __shfl(cslot[g], col, 64);
+        const int rr = __shfl(crr[g], col, 64);
+        const bool ok = __shfl((int)cvalid[g], col, 64) != 0;
+        const int m = mt * 16 * FM + f * 16 + row;
+        if (!ok || m >= n2) continue;
+        const int p = m % no, q = m / no;
+        const int off = cl == 0 ? p + no * (q + no * rr) : p + no * (rr + no * q);
+        double *dst = Gwin + slot * w3 + off;
+        *dst = add ? *dst + v : v;
+      }
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // reads done before the next tile
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
 // One wave tile: rows mt*16*FM .. of the No^2 (p,q) rows, columns
 // nt*16*FN .. of the nb*No (t, r) columns of class cl at A-pair (o1,o2).
 // W of window slot s sits at Gwin + s*No^3 in the order i + No j + No^2 k.
@@ -64,7 +126,7 @@ template <int FM, int FN>
 __device__ __forceinline__ void wave_tile(const BSys &S, int cl, int al,
                                           int be, int ga, int o1, int o2,
                                           int mt, int nt, double *Gwin,
-                                          bool add, int lane) {
+                                          bool add, int lane, double *xp) {
   const int nb = S.nb;
   const int64_t No = S.No, Nv = S.Nv, N2 = No * No, Kv = S.Kv, Ko = S.Ko;
   const int kk = lane >> 4, l16 = lane & 15;
@@ -124,56 +186,58 @@ __device__ __forceinline__ void wave_tile(const BSys &S, int cl, int al,
                                                      acc[f][g], 0, 0, 0); \
   }
   // segments: 0 T3[x1] straight, 1 T3[x2] swapped, 2 H3[x2] straight,
-  //           3 H3[x1] swapped; K runs are multiples of 16
+  //           3 H3[x1] swapped; K runs are multiples of 16.  One pipeline
+  // runs through all four: the load cursor (lsg, lk) moves to the next
+  // segment's pointers when it passes the end of one, so only the first
+  // chunk of the tile waits for memory.  Past the last chunk the cursor
+  // stays on it (a valid address), so every load is unconditional.
+  int lsg = 0;
+  int64_t lk = 0;
+#define SET_PTRS(SG)                                                          \
+  {                                                                           \
+    const int64_t K_ = (SG) < 2 ? Kv : Ko;                                    \
+    const double *Ab = (SG) < 2 ? S.T3 + ((SG) == 0 ? x1 : x2) * N2 * Kv      \
+                                : S.H3 + ((SG) == 2 ? x2 : x1) * N2 * Ko;     \
+    _Pragma("unroll") for (int f = 0; f < FM; f++) ap[f] =                    \
+        Ab + (int64_t)(((SG)&1) ? rowW[f] : rowS[f]) * K_ + 4 * kk;           \
+    _Pragma("unroll") for (int g = 0; g < FN; g++) bp[g] =                    \
+        bbase(S, cl, (SG), ca[g], cb[g], cc[g]) + (int64_t)crr[g] * K_ +      \
+        4 * kk;                                                               \
+  }
+#define ADVANCE()                                                             \
+  {                                                                           \
+    lk += KCHUNK;                                                             \
+    if (lk >= (lsg < 2 ? Kv : Ko)) {                                          \
+      if (lsg == 3) {                                                         \
+        lk -= KCHUNK;                                                         \
+      } else {                                                                \
+        lsg++;                                                                \
+        lk = 0;                                                               \
+        SET_PTRS(lsg);                                                        \
+      }                                                                       \
+    }                                                                         \
+  }
+  SET_PTRS(0);
+  const int nchunks = (int)(2 * (Kv / KCHUNK) + 2 * (Ko / KCHUNK));
+  LOAD(a0, b0, lk);
+  ADVANCE();
 #pragma unroll 1
-  for (int sg = 0; sg < 4; sg++) {
-    const int64_t K_ = sg < 2 ? Kv : Ko;
-    const double *Ab = sg < 2 ? S.T3 + (sg == 0 ? x1 : x2) * N2 * Kv
-                              : S.H3 + (sg == 2 ? x2 : x1) * N2 * Ko;
-#pragma unroll
-    for (int f = 0; f < FM; f++)
-      ap[f] = Ab + (int64_t)((sg & 1) ? rowW[f] : rowS[f]) * K_ + 4 * kk;
-#pragma unroll
-    for (int g = 0; g < FN; g++)
-      bp[g] = bbase(S, cl, sg, ca[g], cb[g], cc[g]) + (int64_t)crr[g] * K_ +
-              4 * kk;
-    const int64_t last = K_ - KCHUNK;
-    LOAD(a0, b0, 0);
-    for (int64_t k0 = 0;; k0 += 2 * KCHUNK) {
-      LOAD(a1, b1, k0 + KCHUNK < last ? k0 + KCHUNK : last);
-      COMPUTE(a0, b0);
-      if (k0 + KCHUNK > last) break;
-      LOAD(a0, b0, k0 + 2 * KCHUNK < last ? k0 + 2 * KCHUNK : last);
-      COMPUTE(a1, b1);
-      if (k0 + 2 * KCHUNK > last) break;
-    }
+  for (int c = 0;; c += 2) {
+    LOAD(a1, b1, lk);
+    ADVANCE();
+    COMPUTE(a0, b0);
+    if (c + 1 >= nchunks) break;
+    LOAD(a0, b0, lk);
+    ADVANCE();
+    COMPUTE(a1, b1);
+    if (c + 2 >= nchunks) break;
   }
 #undef LOAD
 #undef COMPUTE
-
-  // v_mfma_f64_16x16x4f64: D[i][j] in lane j + 16*(i%4), register i/4.
-  // 32-bit offsets: No <= 64, so a window's W spans < 2^27 doubles.
-  const int no = (int)No, n2 = (int)N2;
-#pragma unroll
-  for (int g = 0; g < FN; g++) {
-    if (!cvalid[g]) continue;
-    double *Gs = Gwin + cslot[g] * (n2 * no);
-    const int rr = crr[g];
-#pragma unroll
-    for (int f = 0; f < FM; f++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int m = mt * 16 * FM + f * 16 + 4 * r + kk;
-        if (m >= n2) continue;
-        const int p = m % no, q = m / no;
-        const int off = cl == 0   ? p + no * (q + no * rr)
-                        : cl == 1 ? p + no * (rr + no * q)
-                                  : rr + no * (p + no * q);
-        double *dst = Gs + off;
-        *dst = add ? *dst + acc[f][g][r] : acc[f][g][r];
-        __builtin_amdgcn_sched_barrier(0);
-      }
-  }
+#undef SET_PTRS
+#undef ADVANCE
+  store_tile<FM, FN, 8>(acc, cl, mt, nt, No, cvalid, cslot, crr, Gwin, add,
+                        lane, xp);
 }
 
 // The same wave tile with FP32 products (mixed forms): hi*hi on
@@ -186,7 +250,7 @@ __device__ __forceinline__ void wave_tile_split(const BSys &S, const BSplit &X,
                                                 int cl, int al, int be, int ga,
                                                 int o1, int o2, int mt, int nt,
                                                 double *Gwin, bool add,
-                                                int lane) {
+                                                int lane, double *xp) {
   const int nb = S.nb;
   const int64_t No = S.No, Nv = S.Nv, N2 = No * No, Kv = S.Kv, Ko = S.Ko;
   const int kk = lane >> 4, l16 = lane & 15;
@@ -261,6 +325,9 @@ __device__ __forceinline__ void wave_tile_split(const BSys &S, const BSplit &X,
                                                       acc[f][g], 0, 0, 0);  \
     }                                                                       \
   }
+  // per-segment pipelines: one pipeline across the segments (as in
+  // wave_tile) keeps more pointers live and costs this tile its second wave
+  // per SIMD
 #pragma unroll 1
   for (int sg = 0; sg < 4; sg++) {
     const int64_t K_ = sg < 2 ? Kv : Ko;
@@ -298,29 +365,8 @@ __device__ __forceinline__ void wave_tile_split(const BSys &S, const BSplit &X,
 #undef LOAD
 #undef COMPUTE
 
-  // 16x16 FP32-output MFMAs: D[i][j] in lane j + 16*(i/4), register i%4
-  const int no = (int)No, n2 = (int)N2;
-#pragma unroll
-  for (int g = 0; g < FN; g++) {
-    if (!cvalid[g]) continue;
-    double *Gs = Gwin + cslot[g] * (n2 * no);
-    const int rr = crr[g];
-#pragma unroll
-    for (int f = 0; f < FM; f++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int m = mt * 16 * FM + f * 16 + 4 * kk + r;
-        if (m >= n2) continue;
-        const int p = m % no, q = m / no;
-        const int off = cl == 0   ? p + no * (q + no * rr)
-                        : cl == 1 ? p + no * (rr + no * q)
-                                  : rr + no * (p + no * q);
-        double *dst = Gs + off;
-        const double v = (double)acc[f][g][r];
-        *dst = add ? *dst + v : v;
-        __builtin_amdgcn_sched_barrier(0);
-      }
-  }
+  store_tile<FM, FN, 4>(acc, cl, mt, nt, No, cvalid, cslot, crr, Gwin, add,
+                        lane, xp);
 }
 
 // Energy of tuple (a,b,c) from its W (order i + No j + No^2 k), by one
@@ -401,6 +447,7 @@ __global__ __launch_bounds__(256) void batched_kernel(BatchedLaunch P) {
   __shared__ int task_sh;
   __shared__ double etup[512];
   __shared__ double tsing[4][3][BMAX_NO];
+  __shared__ double xpose[4][16 * 17];  // per-wave tile transposition
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double *Gwin = P.scratch + (int64_t)blockIdx.x * P.scratch_stride;
 #pragma unroll 1
@@ -435,10 +482,11 @@ __global__ __launch_bounds__(256) void batched_kernel(BatchedLaunch P) {
   case V:                                                                  \
     if constexpr (PREC == ATRIP_FP64)                                      \
       wave_tile<bfm(V), bfn(V)>(S, cl, T.al, T.be, T.ga, o1, o2, mt, nt,   \
-                                Gwin, cl > 0, lane);                       \
+                                Gwin, cl > 0, lane, xpose[wave]);          \
     else                                                                   \
       wave_tile_split<bfm(V), bfn(V), PREC == ATRIP_SPLIT_FP32>(           \
-          S, X, cl, T.al, T.be, T.ga, o1, o2, mt, nt, Gwin, cl > 0, lane); \
+          S, X, cl, T.al, T.be, T.ga, o1, o2, mt, nt, Gwin, cl > 0, lane,  \
+          xpose[wave]);                                                    \
     break;
           TILE_CASE(0)
           TILE_CASE(1)
