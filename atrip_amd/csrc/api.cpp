@@ -22,6 +22,16 @@ DevSystem &atrip_amd::get_sys(atrip_hip_ctx *ctx, int sys) {
 
 namespace {
 void *dev_bytes(DevSystem &s, size_t bytes, const char *what);
+// Zeroed tiled bf16x3 copy (and row-swapped copy) of geometry g.
+void alloc_tri(DevSystem &s, TriOp &o, const TriGeom &g, const char *what) {
+  const size_t bytes = (size_t)tri_parts(g) * 2;
+  o.p = (uint16_t *)dev_bytes(s, bytes, what);
+  check(hipMemset(o.p, 0, bytes), what);
+  if (g.swapped) {
+    o.sw = (uint16_t *)dev_bytes(s, bytes, what);
+    check(hipMemset(o.sw, 0, bytes), what);
+  }
+}
 }
 
 // Split-FP32 copies of the GEMM operands, made once per system on the first
@@ -44,6 +54,47 @@ void atrip_amd::ensure_split(atrip_hip_ctx *ctx, DevSystem &s) {
   make(s.H3J, s.sH3J, Nv * N2 * s.Ko);
   check(hipStreamSynchronize(ctx->stream), "split conversion");
   s.split_ready = true;
+}
+
+void atrip_amd::check_precision(const DevSystem &s, int precision) {
+  if (precision != ATRIP_FP64 && precision != ATRIP_SPLIT_FP32 &&
+      precision != ATRIP_FP32 && precision != ATRIP_BF16X3)
+    throw Fail{"unknown precision " + std::to_string(precision)};
+  if (s.cplx && precision != ATRIP_FP64)
+    throw Fail{"complex systems run in FP64 only"};
+  if (precision == ATRIP_FP64 && s.store != 0)
+    throw Fail{"system was uploaded without FP64 operands (operand_store)"};
+  if (precision == ATRIP_SPLIT_FP32 && (s.store == 2 || s.store == 3))
+    throw Fail{"system was uploaded without split operands (operand_store " +
+               std::to_string(s.store) + ")"};
+  if (precision == ATRIP_FP32 && s.store == 3)
+    throw Fail{"system was uploaded with bf16x3 operands only (operand_store 3)"};
+  if (precision == ATRIP_BF16X3 && s.store != 0 && s.store != 3)
+    throw Fail{"system was uploaded without bf16x3 operands (operand_store " +
+               std::to_string(s.store) + ")"};
+}
+
+// bf16x3 copies of the GEMM operands, made once per system on the first
+// bf16x3 run.
+void atrip_amd::ensure_tri(atrip_hip_ctx *ctx, DevSystem &s) {
+  if (s.tri_ready) return;
+  const int64_t No = s.No, Nv = s.Nv;
+  const TriGeom ga = tri_geom_a(No, Nv, s.Kv), gb = tri_geom_b(No, Nv, s.Kv),
+                gh = tri_geom_a(No, Nv, s.Ko), gt = tri_geom_b(No, Nv, s.Ko);
+  auto make = [&](const double *x, TriOp &o, const TriGeom &g) {
+    if (!x) return;
+    alloc_tri(s, o, g, "alloc bf16x3");
+    check(launch_tri(x, o, g, No, ctx->stream), "bf16x3 conversion");
+  };
+  make(s.T3, s.tT3, ga);
+  make(s.V3, s.tV3, gb);
+  make(s.H3, s.tH3, gh);
+  make(s.T2N, s.tT2N, gt);
+  make(s.T2TN, s.tT2TN, gt);
+  make(s.V3J, s.tV3J, gb);
+  make(s.H3J, s.tH3J, gh);
+  check(hipStreamSynchronize(ctx->stream), "bf16x3 conversion");
+  s.tri_ready = true;
 }
 
 namespace {
@@ -79,14 +130,22 @@ uint64_t atrip_amd::system_bytes(int64_t No, int64_t Nv, bool with_j,
                                  int store) {
   const uint64_t Kv = pad16(Nv), Ko = pad16(No), N2 = No * No, Nv2 = Nv * Nv;
   const auto d = [](uint64_t n) { return std::max<uint64_t>(n, 1) * 8; };
-  // doubles operands: 8 bytes per element as FP64, 6 as split (4 + 2),
-  // 4 as FP32 only
+  // doubles operands: 8 bytes per element as FP64, 6 as split (4 + 2), 4
+  // as FP32 only; bf16x3 copies by their tiled geometry
   const auto o = [&](uint64_t n) {
     return store == 2 ? n * 4 : store == 1 ? n * 6 : d(n);
   };
-  uint64_t b = o(Nv * N2 * Kv) + o(Nv2 * No * Kv) + o(Nv * N2 * Ko) +
-               2 * o(Nv2 * No * Ko) + d(Nv2 * N2) + d(Nv * No) + d(No) + d(Nv);
-  if (with_j) b += o(Nv2 * No * Kv) + o(Nv * N2 * Ko);
+  uint64_t b = d(Nv2 * N2) + d(Nv * No) + d(No) + d(Nv);
+  if (store == 3) {
+    b += tri_bytes(tri_geom_a(No, Nv, Kv)) + tri_bytes(tri_geom_b(No, Nv, Kv)) +
+         tri_bytes(tri_geom_a(No, Nv, Ko)) + 2 * tri_bytes(tri_geom_b(No, Nv, Ko));
+    if (with_j)
+      b += tri_bytes(tri_geom_b(No, Nv, Kv)) + tri_bytes(tri_geom_a(No, Nv, Ko));
+  } else {
+    b += o(Nv * N2 * Kv) + o(Nv2 * No * Kv) + o(Nv * N2 * Ko) +
+         2 * o(Nv2 * No * Ko);
+    if (with_j) b += o(Nv2 * No * Kv) + o(Nv * N2 * Ko);
+  }
   b += (uint64_t)No * (No + 1) * (No + 2) / 6 * sizeof(uint32_t);
   b += (Nv + 1) * sizeof(uint64_t);
   return b;
@@ -106,11 +165,16 @@ void alloc_system(atrip_hip_ctx *ctx, DevSystem &s, int64_t No, int64_t Nv,
   s.Ko = pad16(No);
   const int64_t N2 = No * No, Nv2 = Nv * Nv;
   s.store = ctx->operand_store;
-  // doubles operands: FP64, or only their split / FP32 form
-  auto operand = [&](double *&x, DevSystem::Split &sp, int64_t n,
-                     const char *what) {
+  // doubles operands: FP64, or only their split / FP32 / bf16x3 form
+  auto operand = [&](double *&x, DevSystem::Split &sp, TriOp &tp,
+                     const TriGeom &g, const char *what) {
+    const int64_t n = g.no * g.nr * g.K;
     if (s.store == 0) {
       x = dev_alloc(s, n, what);
+      return;
+    }
+    if (s.store == 3) {
+      alloc_tri(s, tp, g, what);
       return;
     }
     sp.h = (float *)dev_bytes(s, (size_t)n * sizeof(float), what);
@@ -118,20 +182,23 @@ void alloc_system(atrip_hip_ctx *ctx, DevSystem &s, int64_t No, int64_t Nv,
       sp.l = (uint16_t *)dev_bytes(s, (size_t)n * 2, what);
     }
   };
-  operand(s.T3, s.sT3, Nv * N2 * s.Kv, "alloc T3");
-  operand(s.V3, s.sV3, Nv2 * No * s.Kv, "alloc V3");
-  operand(s.H3, s.sH3, Nv * N2 * s.Ko, "alloc H3");
-  operand(s.T2N, s.sT2N, Nv2 * No * s.Ko, "alloc T2N");
-  operand(s.T2TN, s.sT2TN, Nv2 * No * s.Ko, "alloc T2TN");
-  s.split_ready = s.store != 0;
+  const TriGeom ga = tri_geom_a(No, Nv, s.Kv), gb = tri_geom_b(No, Nv, s.Kv),
+                gh = tri_geom_a(No, Nv, s.Ko), gt = tri_geom_b(No, Nv, s.Ko);
+  operand(s.T3, s.sT3, s.tT3, ga, "alloc T3");
+  operand(s.V3, s.sV3, s.tV3, gb, "alloc V3");
+  operand(s.H3, s.sH3, s.tH3, gh, "alloc H3");
+  operand(s.T2N, s.sT2N, s.tT2N, gt, "alloc T2N");
+  operand(s.T2TN, s.sT2TN, s.tT2TN, gt, "alloc T2TN");
+  s.split_ready = s.store == 1 || s.store == 2;
+  s.tri_ready = s.store == 3;
   s.VHH = dev_alloc(s, Nv2 * N2, "alloc VHH");
   s.Tai = dev_alloc(s, Nv * No, "alloc Tai");
   s.ei = dev_alloc(s, No, "alloc ei");
   s.ea = dev_alloc(s, Nv, "alloc ea");
   s.has_j = with_j;
   if (with_j) {
-    operand(s.V3J, s.sV3J, Nv2 * No * s.Kv, "alloc V3J");
-    operand(s.H3J, s.sH3J, Nv * N2 * s.Ko, "alloc H3J");
+    operand(s.V3J, s.sV3J, s.tV3J, gb, "alloc V3J");
+    operand(s.H3J, s.sH3J, s.tH3J, gh, "alloc H3J");
   }
   s.cplx = cplx;
   if (cplx) {
@@ -217,15 +284,17 @@ void staged(atrip_hip_ctx *ctx, const double *host, int64_t slab,
   check(hipFree(stage), "free staging");
 }
 
-// A system that keeps only the split / FP32 form of its doubles operands
-// gets each FP64 operand in a temporary device array, converted and freed
-// as soon as it is filled.
+// A system that keeps only the split / FP32 / bf16x3 form of its doubles
+// operands gets each FP64 operand in a temporary device array, converted
+// and freed as soon as it is filled.
 struct TempOperands {
   DevSystem &s;
   hipStream_t st;
   std::vector<std::pair<double **, DevSystem::Split *>> items;
+  std::vector<std::pair<TriOp *, TriGeom>> tris;
   std::vector<int64_t> sizes;
-  void add(double *&x, DevSystem::Split &sp, int64_t n) {
+  void add(double *&x, DevSystem::Split &sp, TriOp &tp, const TriGeom &g) {
+    const int64_t n = g.no * g.nr * g.K;
     if (s.store == 0) return;
     void *p = nullptr;
     check(hipMalloc(&p, (size_t)std::max<int64_t>(n, 1) * sizeof(double)),
@@ -235,17 +304,24 @@ struct TempOperands {
           "clear temporary operand");
     x = (double *)p;
     items.push_back({&x, &sp});
+    tris.push_back({&tp, g});
     sizes.push_back(n);
   }
   void convert() {
     for (size_t q = 0; q < items.size(); q++) {
-      check(launch_split(*items[q].first, *items[q].second, sizes[q], st),
-            "split conversion");
+      if (s.store == 3)
+        check(launch_tri(*items[q].first, *tris[q].first, tris[q].second,
+                         s.No, st),
+              "bf16x3 conversion");
+      else
+        check(launch_split(*items[q].first, *items[q].second, sizes[q], st),
+              "split conversion");
       check(hipStreamSynchronize(st), "split conversion");
       (void)hipFree(*items[q].first);
       *items[q].first = nullptr;
     }
     items.clear();
+    tris.clear();
     sizes.clear();
   }
   ~TempOperands() {
@@ -261,9 +337,9 @@ void upload_host(atrip_hip_ctx *ctx, DevSystem &s, const atrip_sys_desc *d) {
   const int64_t Kv = s.Kv, Ko = s.Ko;
   hipStream_t st = ctx->stream;
   TempOperands tmp{s, st, {}, {}};
-  tmp.add(s.T3, s.sT3, Nv * N2 * Kv);
-  tmp.add(s.T2N, s.sT2N, Nv2 * No * Ko);
-  tmp.add(s.T2TN, s.sT2TN, Nv2 * No * Ko);
+  tmp.add(s.T3, s.sT3, s.tT3, tri_geom_a(No, Nv, Kv));
+  tmp.add(s.T2N, s.sT2N, s.tT2N, tri_geom_b(No, Nv, Ko));
+  tmp.add(s.T2TN, s.sT2TN, s.tT2TN, tri_geom_b(No, Nv, Ko));
   // Tabij {Nv,Nv,No,No}: slowest index j
   staged(ctx, d->Tabij, Nv2 * No, No,
          [&](const double *c, int64_t j0, int64_t cj) {
@@ -306,17 +382,17 @@ void upload_host(atrip_hip_ctx *ctx, DevSystem &s, const atrip_sys_desc *d) {
            });
   };
   tmp.convert();
-  tmp.add(s.H3, s.sH3, Nv * N2 * Ko);
+  tmp.add(s.H3, s.sH3, s.tH3, tri_geom_a(No, Nv, Ko));
   hhh(d->Vijka, s.H3);
   tmp.convert();
-  tmp.add(s.V3, s.sV3, Nv2 * No * Kv);
+  tmp.add(s.V3, s.sV3, s.tV3, tri_geom_b(No, Nv, Kv));
   ppph(d->Vabci, s.V3);
   tmp.convert();
   if (d->Jijka) {
-    tmp.add(s.H3J, s.sH3J, Nv * N2 * Ko);
+    tmp.add(s.H3J, s.sH3J, s.tH3J, tri_geom_a(No, Nv, Ko));
     hhh(d->Jijka, s.H3J);
     tmp.convert();
-    tmp.add(s.V3J, s.sV3J, Nv2 * No * Kv);
+    tmp.add(s.V3J, s.sV3J, s.tV3J, tri_geom_b(No, Nv, Kv));
     ppph(d->Jabci, s.V3J);
     tmp.convert();
   }
@@ -451,17 +527,12 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   const uint64_t n = n_tuples(s.Nv);
   last = std::min(last, n);
   RunResult res;
-  if (precision != ATRIP_FP64 && precision != ATRIP_SPLIT_FP32 &&
-      precision != ATRIP_FP32)
-    throw Fail{"unknown precision " + std::to_string(precision)};
+  check_precision(s, precision);
   const bool g32 = precision != ATRIP_FP64;
-  if (s.cplx && precision != ATRIP_FP64)
-    throw Fail{"complex systems run in FP64 only"};
-  if (precision == ATRIP_FP64 && s.store != 0)
-    throw Fail{"system was uploaded without FP64 operands (operand_store)"};
-  if (precision == ATRIP_SPLIT_FP32 && s.store == 2)
-    throw Fail{"system was uploaded with FP32 operands only (operand_store 2)"};
-  if (g32) ensure_split(ctx, s);
+  if (precision == ATRIP_BF16X3)
+    ensure_tri(ctx, s);
+  else if (g32)
+    ensure_split(ctx, s);
   if (first >= last) return res;
   const int nb = plan.nb;
   const bool full = first == 0 && last == n;
@@ -541,6 +612,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
                             (s.cplx ? 4.0 : 1.0);
   const char *dname = precision == ATRIP_FP64        ? "doubles"
                       : precision == ATRIP_SPLIT_FP32 ? "doubles_mixed"
+                      : precision == ATRIP_BF16X3     ? "doubles_bf16x3"
                                                       : "doubles_fp32";
   std::vector<double> host_slots;
 
@@ -574,6 +646,10 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
         check(launch_doubles_f64(s, b, pass == 1, (double *)Gp, ctx->tile,
                                  ctx->stream),
               "doubles kernel");
+      else if (precision == ATRIP_BF16X3)
+        check(launch_doubles_tri(s, b, pass == 1, (float *)Gp, ctx->tile,
+                                 ctx->stream),
+              "doubles (bf16x3) kernel");
       else
         check(launch_doubles_split(s, b, pass == 1, (float *)Gp,
                                    precision == ATRIP_SPLIT_FP32, ctx->tile,
@@ -914,7 +990,8 @@ int atrip_hip_set_option(atrip_hip_ctx *ctx, const char *key, int64_t value) {
       if (value < -1 || value > 7) throw Fail{"tile must be in -1..7"};
       ctx->tile = (int)value;
     } else if (k == "operand_store") {
-      if (value < 0 || value > 2) throw Fail{"operand_store must be 0, 1 or 2"};
+      if (value < 0 || value > 3)
+        throw Fail{"operand_store must be 0, 1, 2 or 3"};
       ctx->operand_store = (int)value;
     } else if (k == "batch_scratch_mb") {
       if (value < 16 || value > (1 << 18))

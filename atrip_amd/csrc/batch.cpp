@@ -111,7 +111,9 @@ static std::array<int64_t, 3> shape_key(const DevSystem &s, int precision) {
   return {s.No, pad16(s.Nv), precision};
 }
 
-void check_precision(int precision) {
+void check_batched_precision(int precision) {
+  if (precision == ATRIP_BF16X3)
+    throw Fail{"the batched form runs fp64, split_fp32 and fp32 (not bf16x3)"};
   if (precision != ATRIP_FP64 && precision != ATRIP_SPLIT_FP32 &&
       precision != ATRIP_FP32)
     throw Fail{"unknown precision " + std::to_string(precision)};
@@ -151,10 +153,8 @@ BatchTune plan_for(atrip_hip_ctx *ctx, const DevSystem &s, int grid,
 
 BSys make_bsys(atrip_hip_ctx *ctx, DevSystem &s, const BatchTune &p,
                int precision) {
-  if (precision == ATRIP_FP64 && s.store != 0)
-    throw Fail{"system was uploaded without FP64 operands (operand_store)"};
-  if (precision == ATRIP_SPLIT_FP32 && s.store == 2)
-    throw Fail{"system was uploaded with FP32 operands only (operand_store 2)"};
+  check_batched_precision(precision);
+  check_precision(s, precision);
   if (precision != ATRIP_FP64) ensure_split(ctx, s);
   BSys b;
   b.T3 = s.T3; b.V3 = s.V3; b.H3 = s.H3; b.T2N = s.T2N; b.T2TN = s.T2TN;
@@ -299,7 +299,7 @@ int atrip_hip_batch_model(int64_t No, int64_t Nv, int precision,
 int atrip_hip_batch_plan(atrip_hip_ctx *ctx, int sys, int precision, int *nb,
                          int *fm, int *fn, int *measured, double *sec_window) {
   return guarded(ctx, [&] {
-    check_precision(precision);
+    check_batched_precision(precision);
     const DevSystem &s = get_sys(ctx, sys);
     check(hipSetDevice(ctx->device), "hipSetDevice");
     const int grid =
@@ -318,7 +318,7 @@ int atrip_hip_batch_run(atrip_hip_ctx *ctx, const int *systems, int nsys,
   return guarded(ctx, [&] {
     if (nsys < 0 || (nsys > 0 && (!systems || !energies)))
       throw Fail{"bad system list"};
-    check_precision(precision);
+    check_batched_precision(precision);
     check(hipSetDevice(ctx->device), "hipSetDevice");
     const int grid =
         ctx->batch_grid ? ctx->batch_grid : default_grid(ctx, precision);
@@ -378,7 +378,7 @@ int atrip_hip_batch_tune(atrip_hip_ctx *ctx, const int *systems, int nsys,
   return guarded(ctx, [&] {
     if (nsys < 0 || (nsys > 0 && !systems)) throw Fail{"bad system list"};
     if (mode != 0 && mode != 1) throw Fail{"mode must be 0 or 1"};
-    check_precision(precision);
+    check_batched_precision(precision);
     check(hipSetDevice(ctx->device), "hipSetDevice");
     const int grid =
         ctx->batch_grid ? ctx->batch_grid : default_grid(ctx, precision);

@@ -57,7 +57,7 @@ LaunchOut run_tasks(atrip_hip_ctx *ctx, int precision,
                     const std::vector<int32_t> &order, int grid, bool timed);
 
 // Check a precision code of the batched form.
-void check_precision(int precision);
+void check_batched_precision(int precision);
 
 // Neumaier step adding a window's (sum, comp) to a running (sum, comp).
 void neumaier_add(double &sum, double &comp, double s, double c);

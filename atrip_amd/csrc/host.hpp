@@ -42,6 +42,12 @@ DevSystem &get_sys(atrip_hip_ctx *ctx, int sys);
 
 // Split-FP32 copies of a system's doubles operands (made once).
 void ensure_split(atrip_hip_ctx *ctx, DevSystem &s);
+// bf16x3 copies of a system's doubles operands (made once).
+void ensure_tri(atrip_hip_ctx *ctx, DevSystem &s);
+// Throws Fail unless `precision` is known and the system's operand store
+// holds what it reads: FP64 needs store 0, split FP32 0 or 1, FP32 0-2,
+// bf16x3 0 or 3; complex systems run in FP64 only.
+void check_precision(const DevSystem &s, int precision);
 
 // Device bytes of the resident arrays of a system uploaded with operand
 // store mode `store` (what admission charges).

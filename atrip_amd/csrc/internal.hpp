@@ -16,7 +16,31 @@ namespace atrip_amd {
 // Contraction-index padding of the device layouts: every K run of the
 // doubles GEMMs is a multiple of the 16-element K chunk of the kernels.
 constexpr int KCHUNK = 16;
-inline int64_t pad16(int64_t n) { return (n + 15) / 16 * 16; }
+__host__ __device__ inline int64_t pad16(int64_t n) { return (n + 15) / 16 * 16; }
+
+// bf16x3 operands.  An FP64 operand [o][r < nr][k < K] is held as three
+// bf16 parts per element (x ~ b0 + b1 + b2) in 16 x 16 tiles of (R, k),
+// R = o * nrp + r (nrp: nr, or nr padded to 16 so that every o starts a
+// tile row).  Tile (R / 16, k / 16) sits at ((R / 16) (K / 16) + k / 16)
+// * 768 parts; inside it part p of (R, k) sits at 256 p + 4 L + k % 4 with
+// L = R % 16 + 16 ((k % 16) / 4), the lane that feeds (R, k) to the MFMAs,
+// so each part of a 16 x 16 fragment chunk is one contiguous 512 bytes.
+// The A operands (nr = No^2 rows (p, q)) also keep a row-swapped copy whose
+// row p + No q holds row q + No p of the straight one.
+struct TriGeom {
+  int64_t no, nr, nrp, K;
+  bool swapped;  // keeps a row-swapped copy (A operands)
+};
+struct TriOp {
+  uint16_t *p = nullptr, *sw = nullptr;
+};
+__host__ __device__ inline int64_t tri_parts(const TriGeom &g) {
+  return 3 * pad16(g.no * g.nrp) * g.K;
+}
+__host__ __device__ inline int64_t tri_off(int64_t R, int64_t k, int64_t K) {
+  return ((R >> 4) * (K >> 4) + (k >> 4)) * 768 +
+         4 * ((R & 15) + 16 * ((k & 15) >> 2)) + (k & 3);
+}
 
 // Layout of the per-tuple class buffers written by the doubles kernels and
 // read by the energy kernels: each class keeps its GEMM output order,
@@ -72,10 +96,15 @@ struct DevSystem {
   };
   Split sT3, sV3, sH3, sT2N, sT2TN, sV3J, sH3J;
   bool split_ready = false;
-  // Operand store: 0 FP64 (split copies made on the first mixed run),
-  // 1 split FP32 only (h + l, 6 bytes per element), 2 FP32 (h) only.  Modes
-  // 1 and 2 keep no FP64 copy of the doubles operands (T3, V3, H3, T2N, T2TN
-  // and the J arrays).
+  // bf16x3 copies in the tiled layout (TriGeom): T3 and H3 rows padded to
+  // 16 per slice with row-swapped copies, the pair-indexed arrays unpadded.
+  TriOp tT3, tV3, tH3, tT2N, tT2TN, tV3J, tH3J;
+  bool tri_ready = false;
+  // Operand store: 0 FP64 (split or bf16x3 copies made on the first mixed
+  // run of that form), 1 split FP32 only (h + l, 6 bytes per element), 2
+  // FP32 (h) only, 3 bf16x3 only (6 bytes per element).  Modes 1-3 keep no
+  // FP64 copy of the doubles operands (T3, V3, H3, T2N, T2TN and the J
+  // arrays).
   int store = 0;
   uint32_t *points = nullptr;             // packed (i>=j>=k) triples
   int64_t npoints = 0;
@@ -98,6 +127,31 @@ __device__ inline void split_store(double x, float *h, uint16_t *l,
   const float hi = (float)x;
   h[q] = hi;
   if (l) l[q] = to_bf16((float)(x - (double)hi));
+}
+
+// bf16x3 parts of x at part-0 offset `off` of a tiled copy: b0 = bf16(x),
+// b1 = bf16(x - b0), b2 = bf16(x - b0 - b1), each residual formed in FP64,
+// so b0 + b1 + b2 is x within 2^-27 |x|.
+__device__ inline void tri_put(double x, uint16_t *out, int64_t off) {
+  const uint16_t p0 = to_bf16((float)x);
+  const double r1 = x - (double)__uint_as_float((uint32_t)p0 << 16);
+  const uint16_t p1 = to_bf16((float)r1);
+  const double r2 = r1 - (double)__uint_as_float((uint32_t)p1 << 16);
+  out[off] = p0;
+  out[off + 256] = p1;
+  out[off + 512] = to_bf16((float)r2);
+}
+
+// Element q of FP64 operand [o][r][k] into its tiled copy (and the
+// row-swapped one when the geometry keeps it; No = sqrt(nr) there).
+__device__ inline void tri_store(double x, const TriOp &t, const TriGeom &g,
+                                 int64_t No, int64_t q) {
+  const int64_t k = q % g.K, row = q / g.K, r = row % g.nr, o = row / g.nr;
+  tri_put(x, t.p, tri_off(o * g.nrp + r, k, g.K));
+  if (g.swapped) {
+    const int64_t rs = r / No + No * (r % No);
+    tri_put(x, t.sw, tri_off(o * g.nrp + rs, k, g.K));
+  }
 }
 
 // Tuple windows: block triples (alpha <= beta <= gamma) of edge nb.
@@ -189,6 +243,11 @@ hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
                                 bool use_j, float *G, bool corrections,
                                 int tile, hipStream_t st);
 
+// Doubles contractions in bf16x3 (six part products on bf16 MFMA, FP32
+// accumulation, class buffers in FP32).
+hipError_t launch_doubles_tri(const DevSystem &s, const BatchArgs &b,
+                              bool use_j, float *G, int tile, hipStream_t st);
+
 // Class sums: out[slot] = sum(A) (+ sum(B)) as cubes (w_cube per slot), for
 // class-buffer sets A, B (FP64, or FP32 when g32).
 hipError_t launch_wsum(const DevSystem &s, const BatchArgs &b, const void *A,
@@ -232,6 +291,21 @@ hipError_t launch_scale(const double *src, double *dst, int64_t n,
 // Fill a Split copy of n doubles.
 hipError_t launch_split(const double *x, DevSystem::Split &out, int64_t n,
                         hipStream_t st);
+// Fill the bf16x3 copy (and row-swapped copy) of FP64 operand x of
+// geometry g (No: occupied indices, for the swap).
+hipError_t launch_tri(const double *x, const TriOp &out, const TriGeom &g,
+                      int64_t No, hipStream_t st);
+// Geometry of a system's bf16x3 operands: A (T3, H3 and H3J) and the
+// pair-indexed B arrays (V3, T2N, T2TN, V3J), for contraction length K.
+inline TriGeom tri_geom_a(int64_t No, int64_t Nv, int64_t K) {
+  return TriGeom{Nv, No * No, pad16(No * No), K, true};
+}
+inline TriGeom tri_geom_b(int64_t No, int64_t Nv, int64_t K) {
+  return TriGeom{Nv * Nv, No, No, K, false};
+}
+inline uint64_t tri_bytes(const TriGeom &g) {
+  return (uint64_t)tri_parts(g) * 2 * (g.swapped ? 2 : 1);
+}
 
 // ---- batched form: many small resident systems in one persistent launch --
 

@@ -45,8 +45,8 @@ enum SynthTarget { S_T3, S_V3, S_H3, S_T2N, S_T2TN, S_VHH, S_PLAIN };
 // Fill one device array from the generator; element q of the destination is
 // mapped back to the CTF linear index of the tensor it holds.
 __global__ __launch_bounds__(256) void synth_kernel(double *dst, float *sh,
-                                                    uint16_t *sl,
-                                                    int64_t n,
+                                                    uint16_t *sl, TriOp tri,
+                                                    TriGeom tg, int64_t n,
                                                     int target, int tensor,
                                                     uint64_t stream, int64_t No,
                                                     int64_t Nv, int64_t Kv,
@@ -97,20 +97,23 @@ __global__ __launch_bounds__(256) void synth_kernel(double *dst, float *sh,
                                    (uint64_t)idx);
     if (dst) dst[q] = v;
     if (sh) split_store(v, sh, sl, q);
+    if (tri.p) tri_store(v, tri, tg, No, q);
   }
 }
 
-// Fill n elements of a target layout into dst (FP64) and/or the split
-// arrays o (o.h null: none).
-hipError_t synth_fill(double *dst, const DevSystem::Split &o, int64_t n,
-                      SynthTarget target, int tensor, uint64_t seed,
-                      const DevSystem &s, hipStream_t st) {
+// Fill n elements of a target layout into dst (FP64), the split arrays o
+// (o.h null: none) and/or the bf16x3 copy tri of geometry tg (tri.p null:
+// none).
+hipError_t synth_fill(double *dst, const DevSystem::Split &o, const TriOp &tri,
+                      const TriGeom &tg, int64_t n, SynthTarget target,
+                      int tensor, uint64_t seed, const DevSystem &s,
+                      hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const uint64_t stream = atrip_synth_stream(seed, tensor);
   int64_t blocks = (n + 255) / 256;
   if (blocks > 65536L * 16) blocks = 65536L * 16;
   hipLaunchKernelGGL(synth_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
-                     dst, o.h, o.l, n, (int)target, tensor, stream, s.No,
+                     dst, o.h, o.l, tri, tg, n, (int)target, tensor, stream, s.No,
                      s.Nv, s.Kv, s.Ko);
   return hipGetLastError();
 }
@@ -143,28 +146,34 @@ hipError_t launch_synth(DevSystem &s, uint64_t seed, bool with_j,
   const int64_t No = s.No, Nv = s.Nv, N2 = No * No, Nv2 = Nv * Nv;
   hipError_t e;
   const DevSystem::Split none;
-  // doubles operands: FP64 array, or straight into the split arrays when
-  // the system keeps no FP64 copy
-#define SYN(ptr, sp, n, tgt, tensor)                                        \
-  if ((e = synth_fill(s.store ? nullptr : ptr, s.store ? sp : none, n, tgt, \
-                      (tensor) | tensor_flag, seed, s, st)) != hipSuccess)  \
+  const TriOp tnone;
+  const TriGeom ga = tri_geom_a(No, Nv, s.Kv), gb = tri_geom_b(No, Nv, s.Kv),
+                gh = tri_geom_a(No, Nv, s.Ko), gt = tri_geom_b(No, Nv, s.Ko);
+  // doubles operands: FP64 array, or straight into the split arrays or the
+  // bf16x3 copy when the system keeps no FP64 copy
+#define SYN(ptr, sp, tp, g, tgt, tensor)                                    \
+  if ((e = synth_fill(s.store == 0 ? ptr : nullptr,                         \
+                      s.store == 1 || s.store == 2 ? sp : none,             \
+                      s.store == 3 ? tp : tnone, g, (g).no * (g).nr * (g).K, \
+                      tgt, (tensor) | tensor_flag, seed, s, st)) !=         \
+      hipSuccess)                                                           \
     return e;
 #define SYN64(ptr, n, tgt, tensor)                                         \
-  if ((e = synth_fill(ptr, none, n, tgt, (tensor) | tensor_flag, seed, s,  \
-                      st)) != hipSuccess)                                   \
+  if ((e = synth_fill(ptr, none, tnone, ga, n, tgt, (tensor) | tensor_flag, \
+                      seed, s, st)) != hipSuccess)                          \
     return e;
-  SYN(s.T3, s.sT3, Nv * N2 * s.Kv, S_T3, ATRIP_SYN_TABIJ);
-  SYN(s.V3, s.sV3, Nv2 * No * s.Kv, S_V3, ATRIP_SYN_VABCI);
-  SYN(s.H3, s.sH3, Nv * N2 * s.Ko, S_H3, ATRIP_SYN_VIJKA);
-  SYN(s.T2N, s.sT2N, Nv2 * No * s.Ko, S_T2N, ATRIP_SYN_TABIJ);
-  SYN(s.T2TN, s.sT2TN, Nv2 * No * s.Ko, S_T2TN, ATRIP_SYN_TABIJ);
+  SYN(s.T3, s.sT3, s.tT3, ga, S_T3, ATRIP_SYN_TABIJ);
+  SYN(s.V3, s.sV3, s.tV3, gb, S_V3, ATRIP_SYN_VABCI);
+  SYN(s.H3, s.sH3, s.tH3, gh, S_H3, ATRIP_SYN_VIJKA);
+  SYN(s.T2N, s.sT2N, s.tT2N, gt, S_T2N, ATRIP_SYN_TABIJ);
+  SYN(s.T2TN, s.sT2TN, s.tT2TN, gt, S_T2TN, ATRIP_SYN_TABIJ);
   SYN64(s.VHH, Nv2 * N2, S_VHH, ATRIP_SYN_VABIJ);
   SYN64(s.Tai, Nv * No, S_PLAIN, ATRIP_SYN_TAI);
   SYN64(s.ei, No, S_PLAIN, ATRIP_SYN_EI);
   SYN64(s.ea, Nv, S_PLAIN, ATRIP_SYN_EA);
   if (with_j) {
-    SYN(s.V3J, s.sV3J, Nv2 * No * s.Kv, S_V3, ATRIP_SYN_JABCI);
-    SYN(s.H3J, s.sH3J, Nv * N2 * s.Ko, S_H3, ATRIP_SYN_JIJKA);
+    SYN(s.V3J, s.sV3J, s.tV3J, gb, S_V3, ATRIP_SYN_JABCI);
+    SYN(s.H3J, s.sH3J, s.tH3J, gh, S_H3, ATRIP_SYN_JIJKA);
   }
 #undef SYN
 #undef SYN64

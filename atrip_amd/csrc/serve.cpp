@@ -250,7 +250,7 @@ int atrip_hip_serve_create(atrip_hip_ctx *ctx, uint64_t hbm_budget_bytes,
                            atrip_hip_server **out) {
   return guarded(ctx, [&] {
     if (!out) throw Fail{"null argument"};
-    check_precision(precision);
+    check_batched_precision(precision);
     if (hbm_budget_bytes == 0) throw Fail{"HBM budget must be positive"};
     if (max_active < 1) throw Fail{"max_active must be positive"};
     if (!(iter_ms > 0)) throw Fail{"iter_ms must be positive"};

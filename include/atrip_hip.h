@@ -46,7 +46,10 @@ enum atrip_precision {
   ATRIP_FP64 = 0,        /* FP64 MFMA, the plain path */
   ATRIP_SPLIT_FP32 = 1,  /* split FP32: hi*hi on FP32 MFMA + hi*lo + lo*hi
                             corrections, FP32 accumulation, FP64 energy */
-  ATRIP_FP32 = 2         /* hi*hi only on FP32 MFMA, FP64 energy */
+  ATRIP_FP32 = 2,        /* hi*hi only on FP32 MFMA, FP64 energy */
+  ATRIP_BF16X3 = 3       /* every operand as three bf16 parts (24 bits):
+                            six part products on bf16 MFMA, FP32
+                            accumulation, FP64 energy */
 };
 
 /* Create a context on HIP device `device`.  Replaces the device selection
@@ -118,9 +121,10 @@ int atrip_hip_tuple_energies(atrip_hip_ctx *ctx, int sys, int precision,
  *   "timing"  1: time every hot-kernel launch with HIP events
  *   "tile"    doubles-kernel tile variant 0..7, -1 (default) = automatic
  *   "operand_store"  how the next uploads keep the doubles operands:
- *                    0 FP64 (split copies made by the first mixed run),
- *                    1 split FP32 only, 2 FP32 only (no FP64 copy: for
- *                    systems whose FP64 and split copies do not fit together)
+ *                    0 FP64 (split or bf16x3 copies made by the first
+ *                    mixed run of that form), 1 split FP32 only, 2 FP32
+ *                    only, 3 bf16x3 only (no FP64 copy: for systems whose
+ *                    FP64 and mixed copies do not fit together)
  *   "batch_scratch_mb"  device scratch bound of the batched form (MiB)
  *   "batch_grid"        batched persistent grid (0 = occupancy x CUs)
  *   "batch_order"       batched dispatch: 0 longest window first,

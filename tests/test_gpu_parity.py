@@ -169,15 +169,29 @@ MIXED_REL_TUPLE = 1e-5   # per tuple, relative to max |E_abc| (FP32 accumulation
 MIXED_REL_E = 1e-7       # total E(T), relative
 
 
-def test_mixed_precision_against_fp64(hip_ctx, oracle):
-    for no, nv, seed in [(4, 7, 1), (10, 50, 1), (17, 23, 2)]:
+MIXED_FORMS = ["split_fp32", "bf16x3"]
+_ORACLE_RUNS = {}
+
+
+def _oracle_per_tuple(oracle, no, nv, seed):
+    """Synthetic tensors and the oracle's per-tuple run, once per shape."""
+    key = (no, nv, seed)
+    if key not in _ORACLE_RUNS:
         t = oracle.synthetic(no, nv, seed)
-        e_ref, _, pt_ref, _ = oracle.run(no, nv, t, per_tuple=True)
+        _ORACLE_RUNS[key] = (t, oracle.run(no, nv, t, per_tuple=True))
+    return _ORACLE_RUNS[key]
+
+
+@pytest.mark.parametrize("precision", MIXED_FORMS)
+def test_mixed_precision_against_fp64(hip_ctx, oracle, precision):
+    for no, nv, seed in [(4, 7, 1), (10, 50, 1), (17, 23, 2), (32, 20, 3)]:
+        print(precision, no, nv, seed, flush=True)
+        t, (e_ref, _, pt_ref, _) = _oracle_per_tuple(oracle, no, nv, seed)
         s = hip_ctx.upload(no, nv, t)
         try:
-            e, ect = hip_ctx.run(s, "split_fp32")
+            e, ect = hip_ctx.run(s, precision)
             n = oracle.n_tuples(nv)
-            pt = hip_ctx.tuple_energies(s, 0, n, precision="split_fp32")
+            pt = hip_ctx.tuple_energies(s, 0, n, precision=precision)
         finally:
             hip_ctx.release(s)
         assert abs(e - e_ref) <= MIXED_REL_E * abs(e_ref)
@@ -185,18 +199,19 @@ def test_mixed_precision_against_fp64(hip_ctx, oracle):
         assert np.max(np.abs(pt - pt_ref)) <= MIXED_REL_TUPLE * np.max(np.abs(pt_ref))
 
 
-def test_mixed_precision_cT_and_partition(hip_ctx, oracle):
+@pytest.mark.parametrize("precision", MIXED_FORMS)
+def test_mixed_precision_cT_and_partition(hip_ctx, oracle, precision):
     no, nv = 9, 21
     t = oracle.synthetic(no, nv, 5, with_j=True)
     e_ref, ect_ref = oracle.run(no, nv, t)
     s = hip_ctx.upload(no, nv, t)
     try:
-        e, ect = hip_ctx.run(s, "split_fp32")
+        e, ect = hip_ctx.run(s, precision)
         n = oracle.n_tuples(nv)
-        a = hip_ctx.tuple_energies(s, 0, n, precision="split_fp32")
+        a = hip_ctx.tuple_energies(s, 0, n, precision=precision)
         hip_ctx.set_option("window", 4)
         hip_ctx.set_option("batch", 1)
-        b = hip_ctx.tuple_energies(s, 0, n, precision="split_fp32")
+        b = hip_ctx.tuple_energies(s, 0, n, precision=precision)
         hip_ctx.set_option("window", 8)
         hip_ctx.set_option("batch", 4)
     finally:
@@ -206,18 +221,48 @@ def test_mixed_precision_cT_and_partition(hip_ctx, oracle):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", MIXED_FORMS)
+def test_mixed_tiles_and_windows(hip_ctx, oracle, precision):
+    """Every tile variant and window shape gives the same per-tuple bits
+    (each W element accumulates the same chunks in the same order), within
+    the mixed tolerance of the oracle."""
+    no, nv, seed = 12, 40, 78
+    t, (e_ref, _, pt_ref, _) = _oracle_per_tuple(oracle, no, nv, seed)
+    s = hip_ctx.upload(no, nv, t)
+    n = oracle.n_tuples(nv)
+    outs = []
+    try:
+        for window, batch, tile in [(8, 4, 0), (4, 1, 1), (16, 2, 2), (5, 7, 3),
+                                    (8, 2, 4), (3, 5, 5), (8, 1, 6), (6, 3, 7),
+                                    (8, 4, -1)]:
+            print(precision, "window", window, "batch", batch, "tile", tile, flush=True)
+            hip_ctx.set_option("window", window)
+            hip_ctx.set_option("batch", batch)
+            hip_ctx.set_option("tile", tile)
+            outs.append(hip_ctx.tuple_energies(s, 0, n, precision=precision))
+    finally:
+        hip_ctx.set_option("tile", -1)
+        hip_ctx.set_option("window", 8)
+        hip_ctx.set_option("batch", 4)
+        hip_ctx.release(s)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    assert np.max(np.abs(outs[0] - pt_ref)) <= MIXED_REL_TUPLE * np.max(np.abs(pt_ref))
+
+
 def test_operand_store_modes(hip_ctx, oracle):
-    """Systems kept without FP64 operands (split or FP32 only) give the same
-    bits as the split copies of an FP64-resident system; FP64 runs on them
-    are refused."""
+    """Systems kept without FP64 operands (split, FP32 or bf16x3 only) give
+    the same bits as the copies an FP64-resident system makes; runs needing
+    operands the system does not hold are refused."""
     from atrip_amd import hip
 
     no, nv, seed = 7, 19, 11
     t = oracle.synthetic(no, nv, seed, with_j=True)
     ref = hip_ctx.upload_synthetic(no, nv, seed, with_j=True)
     try:
-        want = {p: hip_ctx.run(ref, p) for p in ("split_fp32", "fp32")}
-        for store, precs in ((1, ("split_fp32", "fp32")), (2, ("fp32",))):
+        want = {p: hip_ctx.run(ref, p) for p in ("split_fp32", "fp32", "bf16x3")}
+        for store, precs in ((1, ("split_fp32", "fp32")), (2, ("fp32",)),
+                             (3, ("bf16x3",))):
             hip_ctx.set_option("operand_store", store)
             try:
                 sys_dev = hip_ctx.upload_synthetic(no, nv, seed, with_j=True)
@@ -230,9 +275,15 @@ def test_operand_store_modes(hip_ctx, oracle):
                     assert hip_ctx.run(sys_host, p) == want[p], (store, p)
                 with pytest.raises(hip.HipError, match="FP64 operands"):
                     hip_ctx.run(sys_dev, "fp64")
-                if store == 2:
-                    with pytest.raises(hip.HipError, match="FP32 operands only"):
+                if store >= 2:
+                    with pytest.raises(hip.HipError, match="without split operands"):
                         hip_ctx.run(sys_dev, "split_fp32")
+                if store in (1, 2):
+                    with pytest.raises(hip.HipError, match="without bf16x3 operands"):
+                        hip_ctx.run(sys_dev, "bf16x3")
+                if store == 3:
+                    with pytest.raises(hip.HipError, match="bf16x3 operands only"):
+                        hip_ctx.run(sys_dev, "fp32")
                 assert hip_ctx.sys_info(sys_dev)["bytes"] < hip_ctx.sys_info(ref)["bytes"]
             finally:
                 hip_ctx.release(sys_dev)
