@@ -145,7 +145,15 @@ Atrip::Output run_impl(Atrip::Input<F> const &in, size_t np) {
   if (in.delete_Vppph) delete in.Vppph;  // reference: src/atrip/Atrip.cxx:310
 
   const uint64_t n = (uint64_t)Nv * (Nv + 1) * (Nv + 2) / 6 - Nv;
-  const int precision = in.mixed_precision ? ATRIP_SPLIT_FP32 : ATRIP_FP64;
+  int precision = ATRIP_FP64;
+  if (in.mixed_precision) {
+    if (in.mixed_form == "split_fp32") precision = ATRIP_SPLIT_FP32;
+    else if (in.mixed_form == "fp32") precision = ATRIP_FP32;
+    else if (in.mixed_form == "bf16x3") precision = ATRIP_BF16X3;
+    else
+      throw std::string("atrip: unknown mixed_form '" + in.mixed_form +
+                        "' (split_fp32, fp32 or bf16x3)");
+  }
 
   // checkpoint resume (reference :587-621): the stored energy is physical
   uint64_t first = 0;

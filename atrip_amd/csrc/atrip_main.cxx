@@ -19,7 +19,8 @@ void usage() {
       << "usage: atrip --no N --nv N [--seed S] [--max-iterations K]\n"
          "             [--checkpoint-path P] [--nocheckpoint]\n"
          "             [--checkpoint-it K] [--checkpoint-% P] [--ijkabc]\n"
-         "             [--cT] [--mixed] [--complex] [--device D] [--chrono]\n";
+         "             [--cT] [--mixed] [--mixed-form split_fp32|fp32|bf16x3]\n"
+         "             [--complex] [--device D] [--chrono]\n";
 }
 
 template <typename F>
@@ -60,6 +61,7 @@ struct Options {
   bool nockpt = false, ijkabc = false, ct = false, mixed = false,
        chrono = false;
   std::string ck_path = "atrip-checkpoint.yaml";
+  std::string mixed_form = "split_fp32";
 };
 
 template <typename F>
@@ -100,6 +102,7 @@ int run(const Options &o) {
                 .with_chrono(o.chrono)
                 .with_device(o.device)
                 .with_mixed_precision(o.mixed)
+                .with_mixed_form(o.mixed_form)
                 .with_delete_Vppph(true);
   try {
     const auto out = Atrip::run(in);
@@ -140,6 +143,10 @@ int main(int argc, char **argv) {
     else if (a == "--ijkabc") o.ijkabc = true;
     else if (a == "--cT") o.ct = true;
     else if (a == "--mixed") o.mixed = true;
+    else if (a == "--mixed-form") {
+      o.mixed = true;
+      o.mixed_form = next();
+    }
     else if (a == "--complex") cplx = true;
     else if (a == "--device") o.device = std::atoi(next());
     else if (a == "--chrono") o.chrono = true;

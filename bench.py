@@ -28,7 +28,12 @@ sys.path.insert(0, HERE)
 
 METRIC = ("(T) tuples/s & TFLOP/s, FP64 vs mixed @|ΔE|≤1e-7 Eh; "
           "serve p50/p99")
-PEAK_TFLOPS = {"fp64": 78.6, "split_fp32": 157.3, "fp32": 157.3}
+PEAK_TFLOPS = {"fp64": 78.6, "split_fp32": 157.3, "fp32": 157.3,
+               "bf16x3": 2500.0}
+# MFMA work per algorithmic FLOP: bf16x3 forms six bf16 part products
+MFMA_PER_FLOP = {"fp64": 1.0, "split_fp32": 1.0, "fp32": 1.0, "bf16x3": 6.0}
+KERNEL_NAME = {"fp64": "doubles", "split_fp32": "doubles_mixed",
+               "fp32": "doubles_fp32", "bf16x3": "doubles_bf16x3"}
 
 
 def log(msg: str):
@@ -174,14 +179,14 @@ def bench_serving(ctx, n_requests: int, rate: float, seed: int):
 def bench_c3(ctx, no: int, nv: int, seed: int, a0: int, a1: int, window: int):
     """Plain FP64 vs the mixed forms at No=60 Nv=600 on the tuples whose first
     index is in [a0, a1) (whole windows), each form on its own resident copy
-    (FP64 operands; split-only; FP32-only)."""
+    (FP64 operands; split-only; FP32-only; bf16x3-only)."""
     first = tuples_before(a0, nv)
     count = tuples_before(a1, nv) - first
     flop_tuple = 12.0 * no ** 3 * (no + nv)
     out = {"workload": (f"(T) No={no} Nv={nv} seed {seed}, tuples with first index "
                         f"in [{a0},{a1}) ({count})"), "tuples": count}
     ref = None
-    for prec, store in (("fp64", 0), ("split_fp32", 1), ("fp32", 2)):
+    for prec, store in (("fp64", 0), ("split_fp32", 1), ("fp32", 2), ("bf16x3", 3)):
         log(f"No={no} Nv={nv} sample, {prec}")
         ctx.set_option("operand_store", store)
         try:
@@ -192,8 +197,7 @@ def bench_c3(ctx, no: int, nv: int, seed: int, a0: int, a1: int, window: int):
             ctx.set_option("operand_store", 0)
         try:
             ctx.run(s_, prec, first=first, count=min(count, 1))  # warm
-            kname = {"fp64": "doubles", "split_fp32": "doubles_mixed",
-                     "fp32": "doubles_fp32"}[prec]
+            kname = KERNEL_NAME[prec]
             ctx.reset_stats()
             ctx.set_option("timing", 1)
             t0 = time.perf_counter()
@@ -208,6 +212,9 @@ def bench_c3(ctx, no: int, nv: int, seed: int, a0: int, a1: int, window: int):
                "tflops": count * flop_tuple / dt / 1e12,
                "doubles_kernel_tflops": k.flop / (k.total_ms * 1e-3) / 1e12 if k.total_ms else None,
                "energy_Eh": e, "resident_gb": info["bytes"] / 1e9, "upload_s": up}
+        if k.total_ms:
+            ent["doubles_kernel_mfma_frac"] = (ent["doubles_kernel_tflops"] * MFMA_PER_FLOP[prec]
+                                               / PEAK_TFLOPS[prec])
         if ref is None:
             ref = ent
         else:
@@ -226,7 +233,8 @@ def main(argv=None):
     ap.add_argument("--no", type=int, default=40)
     ap.add_argument("--nv", type=int, default=400)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "split_fp32", "fp32"])
+    ap.add_argument("--precision", default="fp64",
+                    choices=["fp64", "split_fp32", "fp32", "bf16x3"])
     ap.add_argument("--window", type=int, default=8)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--tile", type=int, default=-1)
@@ -236,8 +244,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision comparison pass")
-    ap.add_argument("--mixed", default="split_fp32", choices=["split_fp32", "fp32"],
-                    help="form of the comparison pass")
+    ap.add_argument("--mixed", default="split_fp32,bf16x3",
+                    help="comma-separated forms of the comparison passes "
+                         "(split_fp32, fp32, bf16x3)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true")
@@ -285,8 +294,7 @@ def main(argv=None):
     ctx.set_option("timing", 0)
 
     flop_tuple = 12.0 * args.no ** 3 * (args.no + args.nv)
-    kname = {"fp64": "doubles", "split_fp32": "doubles_mixed",
-             "fp32": "doubles_fp32"}[args.precision]
+    kname = KERNEL_NAME[args.precision]
     ks = ctx.kernel_stats(kname)
     es = ctx.kernel_stats("energy")
     achieved = ks.flop / (ks.total_ms * 1e-3) / 1e12 if ks.total_ms else 0.0
@@ -295,34 +303,38 @@ def main(argv=None):
 
     mixed = None
     if args.precision == "fp64" and not args.no_mixed:
-        # same steps in split FP32 (mixed form) on the same resident inputs
-        ctx.run(sys_, args.mixed, first=slices[-1][0], count=slices[-1][1])
-        ctx.reset_stats()
-        ctx.set_option("timing", 1)
-        m_sum = 0.0
-        t0m = time.perf_counter()
-        for i in range(args.steps):
-            first, count = slices[i % len(slices)]
-            em, _ = ctx.run(sys_, args.mixed, first=first, count=count)
-            m_sum += em
-            log(f"{args.mixed} step {i + 1}/{args.steps}")
-        m_elapsed = time.perf_counter() - t0m
-        ctx.set_option("timing", 0)
-        mk = ctx.kernel_stats("doubles_mixed" if args.mixed == "split_fp32"
-                              else "doubles_fp32")
-        m_value = done / m_elapsed
-        mixed = {
-            "form": args.mixed,
-            "value": m_value,
-            "unit": "tuples/s",
-            "speedup_vs_fp64": m_value / value,
-            "tflops": m_value * flop_tuple / 1e12,
-            "doubles_kernel_tflops": mk.flop / (mk.total_ms * 1e-3) / 1e12 if mk.total_ms else 0.0,
-            "doubles_kernel_frac_of_fp32_peak": (mk.flop / (mk.total_ms * 1e-3) / 1e12 / PEAK_TFLOPS["split_fp32"]) if mk.total_ms else 0.0,
-            "energy_Eh": m_sum,
-            "abs_dE_Eh": abs(m_sum - e_sum),
-            "rel_dE": abs(m_sum - e_sum) / abs(e_sum) if e_sum else None,
-        }
+        # the same steps in each mixed form on the same resident inputs
+        mixed = {}
+        for form in [f for f in args.mixed.split(",") if f]:
+            if form not in ("split_fp32", "fp32", "bf16x3"):
+                raise SystemExit(f"unknown mixed form {form!r}")
+            ctx.run(sys_, form, first=slices[-1][0], count=slices[-1][1])
+            ctx.reset_stats()
+            ctx.set_option("timing", 1)
+            m_sum = 0.0
+            t0m = time.perf_counter()
+            for i in range(args.steps):
+                first, count = slices[i % len(slices)]
+                em, _ = ctx.run(sys_, form, first=first, count=count)
+                m_sum += em
+                log(f"{form} step {i + 1}/{args.steps}")
+            m_elapsed = time.perf_counter() - t0m
+            ctx.set_option("timing", 0)
+            mk = ctx.kernel_stats(KERNEL_NAME[form])
+            m_value = done / m_elapsed
+            k_tf = mk.flop / (mk.total_ms * 1e-3) / 1e12 if mk.total_ms else 0.0
+            mixed[form] = {
+                "value": m_value,
+                "unit": "tuples/s",
+                "speedup_vs_fp64": m_value / value,
+                "tflops": m_value * flop_tuple / 1e12,
+                "doubles_kernel_tflops": k_tf,
+                "doubles_kernel_mfma_frac": k_tf * MFMA_PER_FLOP[form] / PEAK_TFLOPS[form],
+                "mfma_peak_tflops": PEAK_TFLOPS[form],
+                "energy_Eh": m_sum,
+                "abs_dE_Eh": abs(m_sum - e_sum),
+                "rel_dE": abs(m_sum - e_sum) / abs(e_sum) if e_sum else None,
+            }
 
     ctx.release(sys_)
     log("batched form")

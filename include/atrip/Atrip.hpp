@@ -122,7 +122,10 @@ struct Atrip {
     ATRIP_ADD_ATTRIBUTE(size_t, ooo_blocks, 0)   // no effect
     // Additions of this build (no counterpart in the reference):
     ATRIP_ADD_ATTRIBUTE(int, device, 0)          // HIP device index
-    ATRIP_ADD_ATTRIBUTE(bool, mixed_precision, false)  // split-FP32 doubles
+    ATRIP_ADD_ATTRIBUTE(bool, mixed_precision, false)  // mixed doubles:
+    // "split_fp32" (hi*hi + hi*lo + lo*hi on FP32 MFMA), "fp32" (hi*hi only)
+    // or "bf16x3" (six bf16 part products); FP64 energy in every form
+    ATRIP_ADD_ATTRIBUTE(std::string, mixed_form, "split_fp32")
   };
 
   struct Output {
