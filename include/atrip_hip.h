@@ -152,8 +152,9 @@ int atrip_hip_device_count(void);
 
 /*
  * Batched form: the full tuple spaces of nsys resident systems (No <= 64) in
- * one persistent launch, in `precision` (enum atrip_precision; the mixed
- * forms widen each FP32 tile into an FP64 W); energies[s] receives E(T) of
+ * one persistent launch, in `precision` (ATRIP_FP64, ATRIP_SPLIT_FP32 or
+ * ATRIP_FP32; the mixed forms widen each FP32 tile into an FP64 W;
+ * ATRIP_BF16X3 is refused); energies[s] receives E(T) of
  * systems[s] with the sign of src/atrip/Atrip.cxx:1108-1111.  No
  * counterpart in the reference, which runs one system per atrip::Atrip::run
  * (src/atrip/Atrip.cxx:686-1111); each system's result is that of
@@ -195,8 +196,9 @@ int atrip_hip_batch_model(int64_t No, int64_t Nv, int precision,
  * what fits and runs one persistent launch over windows of every active
  * request (shortest remaining work first by default), about iter_ms long,
  * then retires finished requests and frees their memory.  All requests of
- * a server compute in its `precision`; the systems keep only that operand
- * form resident (operand_store 0, 1 or 2).  A request's energy does not
+ * a server compute in its `precision` (ATRIP_FP64, ATRIP_SPLIT_FP32 or
+ * ATRIP_FP32); the systems keep only that operand form resident
+ * (operand_store 0, 1 or 2).  A request's energy does not
  * depend on the mix it was served in.  The server uses its
  * context's stream and must be driven from the context's thread; the
  * context must outlive it.  Errors are reported on the context.
