@@ -16,15 +16,26 @@
 // in the W layout W[i + No j + No^2 k]; the energy kernel sums the three.
 //
 // MFMA v_mfma_f64_16x16x4f64, operands straight from global memory (L2/MALL
-// resident) into registers.  A K chunk of 16 is split as
-// kappa = k0 + 4*(lane/16) + s, s = 0..3, so every lane reads 4 consecutive
-// doubles (32 B) of one row/column per chunk and feeds 4 MFMA k-steps.
+// resident) into registers.  A K chunk of 16 is split over the lane groups
+// q = lane/16 as kappa = k0 + 2q + {0, 1} (k-steps 0, 1) and
+// k0 + 8 + 2q + {0, 1} (k-steps 2, 3): each of a lane's two 16-byte loads
+// then sits next to those of the other lane groups, so one load instruction
+// covers 64 contiguous bytes of each of its 16 rows (two whole 32-byte
+// sectors) and the L1 serves half the sector requests of a 32-byte-per-lane
+// split.
 #include "doubles_common.hpp"
 
 namespace atrip_amd {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+// A lane's four doubles of a chunk: 2q, 2q+1 and 8+2q, 9+2q past p.
+__device__ __forceinline__ d4 load_chunk(const double *p) {
+  const d2 lo = *(const d2 *)p, hi = *(const d2 *)(p + 8);
+  return __builtin_shufflevector(lo, hi, 0, 1, 2, 3);
+}
 
 struct DoublesParams {
   const double *T3, *V3, *H3, *T2N, *T2TN;
@@ -118,9 +129,9 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
     const double *Ab_ = (SG) < 2 ? P.T3 + ((SG) == 0 ? x1 : x2) * N2 * Kv   \
                                  : P.H3 + ((SG) == 2 ? x2 : x1) * N2 * Ko;  \
     _Pragma("unroll") for (int f = 0; f < FM; f++) ap[f] =                  \
-        Ab_ + (((SG)&1) ? rowW[f] : rowS[f]) * K_ + 4 * kk;                 \
+        Ab_ + (((SG)&1) ? rowW[f] : rowS[f]) * K_ + 2 * kk;                 \
     _Pragma("unroll") for (int g = 0; g < FN; g++) bp[g] =                  \
-        b_base(P, cl, (SG), ca[g], cb[g], cc[g]) + crr[g] * K_ + 4 * kk;    \
+        b_base(P, cl, (SG), ca[g], cb[g], cc[g]) + crr[g] * K_ + 2 * kk;    \
   }
   d4 acc[FM][FN];
 #pragma unroll
@@ -136,9 +147,9 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
 #define LOAD(AV, BV, KOFF)                                              \
   {                                                                     \
     _Pragma("unroll") for (int f = 0; f < FM; f++) AV[f] =              \
-        *(const d4 *)(ap[f] + (KOFF));                                  \
+        load_chunk(ap[f] + (KOFF));                                     \
     _Pragma("unroll") for (int g = 0; g < FN; g++) BV[g] =              \
-        *(const d4 *)(bp[g] + (KOFF));                                  \
+        load_chunk(bp[g] + (KOFF));                                     \
     __builtin_amdgcn_sched_barrier(0);                                  \
   }
 #define COMPUTE(AV, BV)                                                 \

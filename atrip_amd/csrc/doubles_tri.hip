@@ -18,11 +18,11 @@
 //     [a1, a0] . [b0, b1]  ->  a1 b0 + a0 b1
 //     [a0, a2] . [b2, b0]  ->  a0 b2 + a2 b0
 // Three bf16 MFMAs (16 cycles each) per 16-deep chunk against four FP64
-// MFMAs (64 cycles each).  Operands are in the tiled layout (TriGeom): the
-// three parts of a 16 x 16 fragment chunk are three contiguous 512-byte
-// runs in lane order, so each part is one fully coalesced 8-byte load per
-// lane; the swapped A segments read the row-swapped copies, so every A
-// fragment is 16 consecutive rows too.  Class buffers are FP32 and are
+// MFMAs (64 cycles each).  Operands are in the tiled layout (TriGeom): parts
+// 0/1 of a 16 x 16 fragment chunk are 1 KiB and part 2 is 512 bytes, both
+// contiguous in lane order, so a fragment chunk is two fully coalesced
+// loads per lane (16 + 8 bytes); the swapped A segments read the
+// row-swapped copies, so every A fragment is 16 consecutive rows too.  Class buffers are FP32 and are
 // widened to FP64 by the class-sum kernel; singles, the energy and all sums
 // stay FP64 (energy.hip).
 #include "doubles_common.hpp"
@@ -34,21 +34,30 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-// A lane's three parts at its four kappa of a chunk.
+// A lane's three parts at its four kappa of a chunk: [b0, b1] and b2.
 struct Parts {
-  bf4 p0, p1, p2;
+  bf8 p01;
+  bf4 p2;
 };
 
-__device__ __forceinline__ Parts load_parts(const __bf16 *p) {
+__device__ __forceinline__ Parts load_parts(const __bf16 *p, int d2) {
   Parts r;
-  r.p0 = *(const bf4 *)p;
-  r.p1 = *(const bf4 *)(p + 256);
-  r.p2 = *(const bf4 *)(p + 512);
+  r.p01 = *(const bf8 *)p;
+  r.p2 = *(const bf4 *)(p + d2);
   return r;
 }
 
-__device__ __forceinline__ bf8 cat(bf4 x, bf4 y) {
-  return __builtin_shufflevector(x, y, 0, 1, 2, 3, 4, 5, 6, 7);
+__device__ __forceinline__ bf8 swap01(bf8 x) {
+  return __builtin_shufflevector(x, x, 4, 5, 6, 7, 0, 1, 2, 3);
+}
+// [x0, y] and [y, x0] for x = [x0, x1]
+__device__ __forceinline__ bf8 with2_a(bf8 x, bf4 y) {
+  const bf8 yy = __builtin_shufflevector(y, y, 0, 1, 2, 3, 0, 1, 2, 3);
+  return __builtin_shufflevector(x, yy, 0, 1, 2, 3, 8, 9, 10, 11);
+}
+__device__ __forceinline__ bf8 with2_b(bf8 x, bf4 y) {
+  const bf8 yy = __builtin_shufflevector(y, y, 0, 1, 2, 3, 0, 1, 2, 3);
+  return __builtin_shufflevector(yy, x, 0, 1, 2, 3, 8, 9, 10, 11);
 }
 
 struct TriParams {
@@ -112,19 +121,25 @@ __global__ __launch_bounds__(256) void doubles_tri_kernel(TriParams P) {
   class_pair(cl, al, be, ga, nb, o1, o2, x1, x2);
   if (x1 >= Nv) x1 = Nv - 1;
   if (x2 >= Nv) x2 = Nv - 1;
-  int32_t rowS[FM];  // the swapped segments read the same rows of the
-                     // row-swapped copies
+  // the swapped segments read the same rows of the row-swapped copies.
+  // Rows past the padded slice (never stored) are folded back into its
+  // last tile at the same position mod 16, so every lane of an A fragment
+  // sits at its own lane of the tile (L = lane) and reads in bounds.
+  const int64_t Kv = P.Kv, Ko = P.Ko, N2p = P.N2p;
+  int32_t rowS[FM];
 #pragma unroll
   for (int f = 0; f < FM; f++) {
     int64_t m = (int64_t)mt * MT + wm * (MT / 2) + f * 16 + l16;
-    if (m >= N2) m = N2 - 1;
+    if (m >= N2p) m = N2p - 16 + (m & 15);
     rowS[f] = (int32_t)m;
   }
 
-  const int64_t Kv = P.Kv, Ko = P.Ko, N2p = P.N2p;
-  // a lane's parts of chunk k0 start at tri_off(R, 4 kk, K) + 48 k0 (the
-  // next tile of the same rows); its three parts are 256 apart
+  // a lane's parts of chunk k0 start at tri_pos(R, 4 kk, K) + 48 k0 (the
+  // next tile of the same rows); part 2 is d2 parts past parts 0/1 (the
+  // lane constant 512 - 4 lane for A, per column for B)
+  const int dA = 512 - 4 * lane;
   const __bf16 *ap[FM], *bp[FN];
+  int32_t dB[FN];
 #define SET_PTRS(SG)                                                          \
   {                                                                           \
     const int64_t K_ = (SG) < 2 ? Kv : Ko;                                    \
@@ -134,37 +149,37 @@ __global__ __launch_bounds__(256) void doubles_tri_kernel(TriParams P) {
                        : (SG) == 2 ? P.H3                                     \
                                    : P.H3s;                                   \
     _Pragma("unroll") for (int f = 0; f < FM; f++) ap[f] =                    \
-        A_ + tri_off(xa_ * N2p + rowS[f], 4 * kk, K_);                        \
+        A_ + tri_pos(xa_ * N2p + rowS[f], 4 * kk, K_).p01;                    \
     _Pragma("unroll") for (int g = 0; g < FN; g++) {                          \
       int64_t u_, v_;                                                         \
       const int wh_ = b_pair(cl, (SG), ca[g], cb[g], cc[g], u_, v_);          \
       const __bf16 *B_ = wh_ == 0 ? P.V3 : wh_ == 1 ? P.T2N : P.T2TN;         \
-      bp[g] = B_ + tri_off((u_ + Nv * v_) * No + crr[g], 4 * kk, K_);         \
+      const TriPos q_ = tri_pos((u_ + Nv * v_) * No + crr[g], 4 * kk, K_);    \
+      bp[g] = B_ + q_.p01;                                                    \
+      dB[g] = (int32_t)(q_.p2 - q_.p01);                                      \
     }                                                                         \
   }
 #define LOAD(A, B, KOFF)                                                     \
   {                                                                          \
     _Pragma("unroll") for (int f = 0; f < FM; f++) A[f] =                    \
-        load_parts(ap[f] + 48 * (KOFF));                                     \
+        load_parts(ap[f] + 48 * (KOFF), dA);                                 \
     _Pragma("unroll") for (int g = 0; g < FN; g++) B[g] =                    \
-        load_parts(bp[g] + 48 * (KOFF));                                     \
+        load_parts(bp[g] + 48 * (KOFF), dB[g]);                              \
     __builtin_amdgcn_sched_barrier(0);                                       \
   }
+  // [a0,a1].[b0,b1] + [a1,a0].[b0,b1] + [a0,a2].[b2,b0]
 #define COMPUTE(A, B)                                                        \
   {                                                                          \
-    bf8 b01_[FN], b20_[FN];                                                  \
-    _Pragma("unroll") for (int g = 0; g < FN; g++) {                         \
-      b01_[g] = cat(B[g].p0, B[g].p1);                                       \
-      b20_[g] = cat(B[g].p2, B[g].p0);                                       \
-    }                                                                        \
+    bf8 b20_[FN];                                                            \
+    _Pragma("unroll") for (int g = 0; g < FN; g++) b20_[g] =                 \
+        with2_b(B[g].p01, B[g].p2);                                          \
     _Pragma("unroll") for (int f = 0; f < FM; f++) {                         \
-      const bf8 a01_ = cat(A[f].p0, A[f].p1), a10_ = cat(A[f].p1, A[f].p0),  \
-                a02_ = cat(A[f].p0, A[f].p2);                                \
+      const bf8 a10_ = swap01(A[f].p01), a02_ = with2_a(A[f].p01, A[f].p2);  \
       _Pragma("unroll") for (int g = 0; g < FN; g++) {                       \
         acc[f][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(                 \
-            a01_, b01_[g], acc[f][g], 0, 0, 0);                              \
+            A[f].p01, B[g].p01, acc[f][g], 0, 0, 0);                         \
         acc[f][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(                 \
-            a10_, b01_[g], acc[f][g], 0, 0, 0);                              \
+            a10_, B[g].p01, acc[f][g], 0, 0, 0);                             \
         acc[f][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(                 \
             a02_, b20_[g], acc[f][g], 0, 0, 0);                              \
       }                                                                      \
@@ -178,7 +193,10 @@ __global__ __launch_bounds__(256) void doubles_tri_kernel(TriParams P) {
     for (int g = 0; g < FN; g++) acc[f][g] = f4{0.f, 0.f, 0.f, 0.f};
 
   // three operand sets in rotation: two chunks of loads stay in flight
-  // behind the MFMAs of the current one
+  // behind the MFMAs of the current one.  A chunk is 2 (FM + FN) loads, so
+  // three chunks (60 at 5 x 5) fit the 63 of the vector-memory counter and
+  // the wait before a chunk's MFMAs covers that chunk's loads only.
+  static_assert(6 * (FM + FN) <= 63, "three chunks of loads must fit vmcnt");
   Parts a0[FM], b0[FN], a1[FM], b1[FN], a2[FM], b2[FN];
 #define CLAMP(K_) ((K_) < last ? (K_) : last)
 #pragma unroll 1

@@ -22,9 +22,11 @@ __host__ __device__ inline int64_t pad16(int64_t n) { return (n + 15) / 16 * 16;
 // bf16 parts per element (x ~ b0 + b1 + b2) in 16 x 16 tiles of (R, k),
 // R = o * nrp + r (nrp: nr, or nr padded to 16 so that every o starts a
 // tile row).  Tile (R / 16, k / 16) sits at ((R / 16) (K / 16) + k / 16)
-// * 768 parts; inside it part p of (R, k) sits at 256 p + 4 L + k % 4 with
-// L = R % 16 + 16 ((k % 16) / 4), the lane that feeds (R, k) to the MFMAs,
-// so each part of a 16 x 16 fragment chunk is one contiguous 512 bytes.
+// * 768 parts.  With L = R % 16 + 16 ((k % 16) / 4), the lane that feeds
+// (R, k) to the MFMAs, and s = k % 4, parts 0 and 1 sit at 8 L + s and
+// 8 L + 4 + s (16 bytes per lane: [b0 x4, b1 x4]) and part 2 at
+// 512 + 4 L + s, so a 16 x 16 fragment chunk is one contiguous 1 KiB of
+// parts 0/1 and one contiguous 512 bytes of part 2 in lane order.
 // The A operands (nr = No^2 rows (p, q)) also keep a row-swapped copy whose
 // row p + No q holds row q + No p of the straight one.
 struct TriGeom {
@@ -37,9 +39,14 @@ struct TriOp {
 __host__ __device__ inline int64_t tri_parts(const TriGeom &g) {
   return 3 * pad16(g.no * g.nrp) * g.K;
 }
-__host__ __device__ inline int64_t tri_off(int64_t R, int64_t k, int64_t K) {
-  return ((R >> 4) * (K >> 4) + (k >> 4)) * 768 +
-         4 * ((R & 15) + 16 * ((k & 15) >> 2)) + (k & 3);
+// Offsets (in parts) of part 0 (part 1 four further) and part 2 of (R, k).
+struct TriPos {
+  int64_t p01, p2;
+};
+__host__ __device__ inline TriPos tri_pos(int64_t R, int64_t k, int64_t K) {
+  const int64_t t = ((R >> 4) * (K >> 4) + (k >> 4)) * 768;
+  const int64_t L = (R & 15) + 16 * ((k & 15) >> 2);
+  return TriPos{t + 8 * L + (k & 3), t + 512 + 4 * L + (k & 3)};
 }
 
 // Layout of the per-tuple class buffers written by the doubles kernels and
@@ -129,17 +136,17 @@ __device__ inline void split_store(double x, float *h, uint16_t *l,
   if (l) l[q] = to_bf16((float)(x - (double)hi));
 }
 
-// bf16x3 parts of x at part-0 offset `off` of a tiled copy: b0 = bf16(x),
+// bf16x3 parts of x at position `at` of a tiled copy: b0 = bf16(x),
 // b1 = bf16(x - b0), b2 = bf16(x - b0 - b1), each residual formed in FP64,
 // so b0 + b1 + b2 is x within 2^-27 |x|.
-__device__ inline void tri_put(double x, uint16_t *out, int64_t off) {
+__device__ inline void tri_put(double x, uint16_t *out, const TriPos &at) {
   const uint16_t p0 = to_bf16((float)x);
   const double r1 = x - (double)__uint_as_float((uint32_t)p0 << 16);
   const uint16_t p1 = to_bf16((float)r1);
   const double r2 = r1 - (double)__uint_as_float((uint32_t)p1 << 16);
-  out[off] = p0;
-  out[off + 256] = p1;
-  out[off + 512] = to_bf16((float)r2);
+  out[at.p01] = p0;
+  out[at.p01 + 4] = p1;
+  out[at.p2] = to_bf16((float)r2);
 }
 
 // Element q of FP64 operand [o][r][k] into its tiled copy (and the
@@ -147,10 +154,10 @@ __device__ inline void tri_put(double x, uint16_t *out, int64_t off) {
 __device__ inline void tri_store(double x, const TriOp &t, const TriGeom &g,
                                  int64_t No, int64_t q) {
   const int64_t k = q % g.K, row = q / g.K, r = row % g.nr, o = row / g.nr;
-  tri_put(x, t.p, tri_off(o * g.nrp + r, k, g.K));
+  tri_put(x, t.p, tri_pos(o * g.nrp + r, k, g.K));
   if (g.swapped) {
     const int64_t rs = r / No + No * (r % No);
-    tri_put(x, t.sw, tri_off(o * g.nrp + rs, k, g.K));
+    tri_put(x, t.sw, tri_pos(o * g.nrp + rs, k, g.K));
   }
 }
 
