@@ -164,16 +164,30 @@ def bench_batched(ctx, n_systems: int, seed: int):
     }
 
 
-def bench_serving(ctx, n_requests: int, rate: float, seed: int):
+def bench_serving(ctx, n_requests: int, rate: float, seed: int,
+                  precisions=("fp64", "split_fp32")):
+    """The same Poisson request stream served once per precision; the mixed
+    runs also report each request's relative energy difference to FP64."""
     from atrip_amd import serve
 
-    r = serve.run_poisson(ctx, n_requests, rate=rate, seed=seed,
-                          hbm_budget_gb=200.0, iter_ms=100.0, log=log)
-    r.pop("energies")
-    r["workload"] = (f"{n_requests} requests, Poisson {rate} req/s, No log-U[8,40], "
-                     f"Nv log-U[40,320], seed {seed}, tensor seeds 5000+r; "
-                     "HBM budget 200 GB; FP64")
-    return r
+    out = {}
+    ref = None
+    for prec in precisions:
+        log(f"serving form, {prec}")
+        r = serve.run_poisson(ctx, n_requests, rate=rate, seed=seed,
+                              hbm_budget_gb=200.0, iter_ms=100.0, precision=prec,
+                              log=log)
+        e = r.pop("energies")
+        r["workload"] = (f"{n_requests} requests, Poisson {rate} req/s, No log-U[8,40], "
+                         f"Nv log-U[40,320], seed {seed}, tensor seeds 5000+r; "
+                         f"HBM budget 200 GB; {prec}")
+        if ref is None:
+            ref = e
+        else:
+            rel = [abs(e[q] - ref[q]) / abs(ref[q]) for q in e if q in ref and ref[q]]
+            r["max_rel_dE_vs_fp64"] = max(rel) if rel else None
+        out[prec] = r
+    return out
 
 
 def bench_c3(ctx, no: int, nv: int, seed: int, a0: int, a1: int, window: int):
