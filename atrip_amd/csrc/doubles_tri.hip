@@ -7,57 +7,39 @@
 // which carry 24 significant bits, as many as an FP32 value (each part is
 // at most 2^-9 of the one before).  A product is formed from the six part
 // products down to the 2^-18 level,
-//     a0 b0 + (a0 b1 + a1 b0) + (a1 b1 + a0 b2 + a2 b0),
-// on v_mfma_f32_16x16x32_bf16 with FP32 accumulation; the three dropped
+//     a0 b0 + a0 b1 + a1 b0 + a1 b1 + a0 b2 + a2 b0,
+// on v_mfma_f32_32x32x16_bf16 with FP32 accumulation; the three dropped
 // terms (a1 b2, a2 b1, a2 b2) are at most 2^-27 of the product, below an
-// FP32 rounding (2^-24).  For a K chunk of 16, lane group q (lanes 16q..16q+15)
-// holds kappa = k0 + 4q + s, s < 4, of its row/column; each MFMA's 8 K
-// slots of a lane group take two parts at those four kappa, so its K = 32
-// covers two part products of the chunk:
-//     [a0, a1] . [b0, b1]  ->  a0 b0 + a1 b1
-//     [a1, a0] . [b0, b1]  ->  a1 b0 + a0 b1
-//     [a0, a2] . [b2, b0]  ->  a0 b2 + a2 b0
-// Three bf16 MFMAs (16 cycles each) per 16-deep chunk against four FP64
-// MFMAs (64 cycles each).  Operands are in the tiled layout (TriGeom): parts
-// 0/1 of a 16 x 16 fragment chunk are 1 KiB and part 2 is 512 bytes, both
-// contiguous in lane order, so a fragment chunk is two fully coalesced
-// loads per lane (16 + 8 bytes); the swapped A segments read the
-// row-swapped copies, so every A fragment is 16 consecutive rows too.  Class buffers are FP32 and are
-// widened to FP64 by the class-sum kernel; singles, the energy and all sums
-// stay FP64 (energy.hip).
+// FP32 rounding (2^-24).  Each MFMA takes one part of a 32-row A fragment
+// and one part of a 32-column B fragment over a whole 16-deep K chunk (lane
+// l feeds row/column l % 32 at kappa k0 + 8 (l / 32) + 0..7), so a chunk is
+// six MFMAs per fragment pair (6 x 32 cycles per 32 x 32 block, against
+// 16 x 64 cycles in FP64) and the operands are the loaded registers as
+// they are.  Operands are in the tiled layout (TriGeom): each part of a
+// 32 x 16 fragment chunk is 1 KiB contiguous in lane order, one 16-byte load
+// per lane; the swapped A segments read the row-swapped copies, so every A
+// fragment is 32 consecutive rows.  Class buffers are FP32 and are widened
+// to FP64 by the class-sum kernel; singles, the energy and all sums stay
+// FP64 (energy.hip).
 #include "doubles_common.hpp"
 
 namespace atrip_amd {
 namespace {
 
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-// A lane's three parts at its four kappa of a chunk: [b0, b1] and b2.
+// A lane's three parts at its eight kappa of a chunk.
 struct Parts {
-  bf8 p01;
-  bf4 p2;
+  bf8 p0, p1, p2;
 };
 
-__device__ __forceinline__ Parts load_parts(const __bf16 *p, int d2) {
+__device__ __forceinline__ Parts load_parts(const __bf16 *p) {
   Parts r;
-  r.p01 = *(const bf8 *)p;
-  r.p2 = *(const bf4 *)(p + d2);
+  r.p0 = *(const bf8 *)p;
+  r.p1 = *(const bf8 *)(p + 512);
+  r.p2 = *(const bf8 *)(p + 1024);
   return r;
-}
-
-__device__ __forceinline__ bf8 swap01(bf8 x) {
-  return __builtin_shufflevector(x, x, 4, 5, 6, 7, 0, 1, 2, 3);
-}
-// [x0, y] and [y, x0] for x = [x0, x1]
-__device__ __forceinline__ bf8 with2_a(bf8 x, bf4 y) {
-  const bf8 yy = __builtin_shufflevector(y, y, 0, 1, 2, 3, 0, 1, 2, 3);
-  return __builtin_shufflevector(x, yy, 0, 1, 2, 3, 8, 9, 10, 11);
-}
-__device__ __forceinline__ bf8 with2_b(bf8 x, bf4 y) {
-  const bf8 yy = __builtin_shufflevector(y, y, 0, 1, 2, 3, 0, 1, 2, 3);
-  return __builtin_shufflevector(yy, x, 0, 1, 2, 3, 8, 9, 10, 11);
 }
 
 struct TriParams {
@@ -72,9 +54,46 @@ struct TriParams {
   int64_t total, per_xcd;
 };
 
+// Wave tiles of FM x FN fragments of 32 x 32 (2 x 2 waves per workgroup),
+// variant t of the "tile" option.
+constexpr int N_TRI_TILES = 8;
+__host__ __device__ inline int tri_fm(int t) {
+  const int v[N_TRI_TILES] = {2, 2, 3, 2, 4, 3, 2, 5};
+  return v[t];
+}
+__host__ __device__ inline int tri_fn(int t) {
+  const int v[N_TRI_TILES] = {2, 3, 2, 4, 2, 3, 5, 2};
+  return v[t];
+}
+// Relative rate of each variant (operand bytes per MFMA cycle fall with
+// FM FN / (FM + FN)).
+inline double tri_rate(int t) {
+  const double fm = tri_fm(t), fn = tri_fn(t);
+  return fm * fn / (fm + fn);
+}
+// Automatic variant: rate times the useful fraction of the padded
+// M = No^2 by N = nb No tile grid.
+inline int pick_tri_tile(int tile, int64_t No, int nb) {
+  if (tile >= 0 && tile < N_TRI_TILES) return tile;
+  double best = -1.0;
+  int pick = 0;
+  const double M = (double)No * No, N = (double)nb * No;
+  for (int t = 0; t < N_TRI_TILES; t++) {
+    const double mt = 64.0 * tri_fm(t), nt = 64.0 * tri_fn(t);
+    const double use = M / (__builtin_ceil(M / mt) * mt) *
+                       N / (__builtin_ceil(N / nt) * nt);
+    const double score = tri_rate(t) * use;
+    if (score > best) {
+      best = score;
+      pick = t;
+    }
+  }
+  return pick;
+}
+
 template <int FM, int FN>
 __global__ __launch_bounds__(256) void doubles_tri_kernel(TriParams P) {
-  constexpr int MT = 32 * FM, NT = 32 * FN;
+  constexpr int MT = 64 * FM, NT = 64 * FN;
   const int64_t bid = blockIdx.x;
   const int64_t logical = (bid % 8) * P.per_xcd + bid / 8;  // XCD-aware
   if (logical >= P.total) return;
@@ -90,7 +109,7 @@ __global__ __launch_bounds__(256) void doubles_tri_kernel(TriParams P) {
   const int o1 = pair / nb, o2 = pair % nb;
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int kk = lane >> 4, l16 = lane & 15;
+  const int kh = lane >> 5, l32 = lane & 31;
   const int wm = wave >> 1, wn = wave & 1;
   const int64_t No = P.No, N2 = P.N2, Nv = P.Nv;
 
@@ -99,7 +118,7 @@ __global__ __launch_bounds__(256) void doubles_tri_kernel(TriParams P) {
   bool any = false;
 #pragma unroll
   for (int g = 0; g < FN; g++) {
-    const int64_t n = (int64_t)nt * NT + wn * (NT / 2) + g * 16 + l16;
+    const int64_t n = (int64_t)nt * NT + wn * (NT / 2) + g * 32 + l32;
     const int t = (int)(n / No);
     crr[g] = (int32_t)(n % No);
     int64_t a, b, c, slot;
@@ -123,23 +142,21 @@ __global__ __launch_bounds__(256) void doubles_tri_kernel(TriParams P) {
   if (x2 >= Nv) x2 = Nv - 1;
   // the swapped segments read the same rows of the row-swapped copies.
   // Rows past the padded slice (never stored) are folded back into its
-  // last tile at the same position mod 16, so every lane of an A fragment
-  // sits at its own lane of the tile (L = lane) and reads in bounds.
+  // last tile at the same position mod 32, so every lane of an A fragment
+  // sits at its own lane slot of the tile and reads in bounds.
   const int64_t Kv = P.Kv, Ko = P.Ko, N2p = P.N2p;
   int32_t rowS[FM];
 #pragma unroll
   for (int f = 0; f < FM; f++) {
-    int64_t m = (int64_t)mt * MT + wm * (MT / 2) + f * 16 + l16;
-    if (m >= N2p) m = N2p - 16 + (m & 15);
+    int64_t m = (int64_t)mt * MT + wm * (MT / 2) + f * 32 + l32;
+    if (m >= N2p) m = N2p - 32 + (m & 31);
     rowS[f] = (int32_t)m;
   }
 
-  // a lane's parts of chunk k0 start at tri_pos(R, 4 kk, K) + 48 k0 (the
-  // next tile of the same rows); part 2 is d2 parts past parts 0/1 (the
-  // lane constant 512 - 4 lane for A, per column for B)
-  const int dA = 512 - 4 * lane;
+  // a lane's parts of chunk k0 start at its slot of the tile of k0: the
+  // tile of k0 = 0 plus 96 k0 (1536 parts per 16-deep chunk); A lanes sit
+  // at slot lane (aligned rows), B lanes at the slot of their row
   const __bf16 *ap[FM], *bp[FN];
-  int32_t dB[FN];
 #define SET_PTRS(SG)                                                          \
   {                                                                           \
     const int64_t K_ = (SG) < 2 ? Kv : Ko;                                    \
@@ -149,54 +166,49 @@ __global__ __launch_bounds__(256) void doubles_tri_kernel(TriParams P) {
                        : (SG) == 2 ? P.H3                                     \
                                    : P.H3s;                                   \
     _Pragma("unroll") for (int f = 0; f < FM; f++) ap[f] =                    \
-        A_ + tri_pos(xa_ * N2p + rowS[f], 4 * kk, K_).p01;                    \
+        A_ + tri_tile(xa_ * N2p + rowS[f], 0, K_) + 8 * lane;                 \
     _Pragma("unroll") for (int g = 0; g < FN; g++) {                          \
       int64_t u_, v_;                                                         \
       const int wh_ = b_pair(cl, (SG), ca[g], cb[g], cc[g], u_, v_);          \
       const __bf16 *B_ = wh_ == 0 ? P.V3 : wh_ == 1 ? P.T2N : P.T2TN;         \
-      const TriPos q_ = tri_pos((u_ + Nv * v_) * No + crr[g], 4 * kk, K_);    \
-      bp[g] = B_ + q_.p01;                                                    \
-      dB[g] = (int32_t)(q_.p2 - q_.p01);                                      \
+      const int64_t R_ = (u_ + Nv * v_) * No + crr[g];                        \
+      bp[g] = B_ + tri_tile(R_, 0, K_) + 8 * tri_lane(R_, 8 * kh);            \
     }                                                                         \
   }
 #define LOAD(A, B, KOFF)                                                     \
   {                                                                          \
     _Pragma("unroll") for (int f = 0; f < FM; f++) A[f] =                    \
-        load_parts(ap[f] + 48 * (KOFF), dA);                                 \
+        load_parts(ap[f] + 96 * (KOFF));                                     \
     _Pragma("unroll") for (int g = 0; g < FN; g++) B[g] =                    \
-        load_parts(bp[g] + 48 * (KOFF), dB[g]);                              \
+        load_parts(bp[g] + 96 * (KOFF));                                     \
     __builtin_amdgcn_sched_barrier(0);                                       \
   }
-  // [a0,a1].[b0,b1] + [a1,a0].[b0,b1] + [a0,a2].[b2,b0]
+  // a0 b0, a0 b1, a1 b0, a1 b1, a0 b2, a2 b0: each term over all blocks
+  // before the next, so consecutive MFMAs update different accumulators
+#define TERM(A, B, AP, BP)                                                   \
+  _Pragma("unroll") for (int f = 0; f < FM; f++)                             \
+      _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =             \
+          __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[f].AP, B[g].BP,          \
+                                                  acc[f][g], 0, 0, 0);
 #define COMPUTE(A, B)                                                        \
   {                                                                          \
-    bf8 b20_[FN];                                                            \
-    _Pragma("unroll") for (int g = 0; g < FN; g++) b20_[g] =                 \
-        with2_b(B[g].p01, B[g].p2);                                          \
-    _Pragma("unroll") for (int f = 0; f < FM; f++) {                         \
-      const bf8 a10_ = swap01(A[f].p01), a02_ = with2_a(A[f].p01, A[f].p2);  \
-      _Pragma("unroll") for (int g = 0; g < FN; g++) {                       \
-        acc[f][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(                 \
-            A[f].p01, B[g].p01, acc[f][g], 0, 0, 0);                         \
-        acc[f][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(                 \
-            a10_, B[g].p01, acc[f][g], 0, 0, 0);                             \
-        acc[f][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(                 \
-            a02_, b20_[g], acc[f][g], 0, 0, 0);                              \
-      }                                                                      \
-    }                                                                        \
+    TERM(A, B, p0, p0) TERM(A, B, p0, p1) TERM(A, B, p1, p0)                 \
+    TERM(A, B, p1, p1) TERM(A, B, p0, p2) TERM(A, B, p2, p0)                 \
   }
 
-  f4 acc[FM][FN];
+  f16v acc[FM][FN];
 #pragma unroll
   for (int f = 0; f < FM; f++)
 #pragma unroll
-    for (int g = 0; g < FN; g++) acc[f][g] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int g = 0; g < FN; g++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[f][g][r] = 0.f;
 
   // three operand sets in rotation: two chunks of loads stay in flight
-  // behind the MFMAs of the current one.  A chunk is 2 (FM + FN) loads, so
-  // three chunks (60 at 5 x 5) fit the 63 of the vector-memory counter and
-  // the wait before a chunk's MFMAs covers that chunk's loads only.
-  static_assert(6 * (FM + FN) <= 63, "three chunks of loads must fit vmcnt");
+  // behind the MFMAs of the current one.  A chunk is 3 (FM + FN) loads, so
+  // three chunks fit the 63 of the vector-memory counter and the wait
+  // before a chunk's MFMAs covers that chunk's loads only.
+  static_assert(9 * (FM + FN) <= 63, "three chunks of loads must fit vmcnt");
   Parts a0[FM], b0[FN], a1[FM], b1[FN], a2[FM], b2[FN];
 #define CLAMP(K_) ((K_) < last ? (K_) : last)
 #pragma unroll 1
@@ -220,9 +232,11 @@ __global__ __launch_bounds__(256) void doubles_tri_kernel(TriParams P) {
 #undef CLAMP
 #undef SET_PTRS
 #undef LOAD
+#undef TERM
 #undef COMPUTE
 
-  // 16x16 FP32-output MFMAs: D[i][j] in lane j + 16*(i/4), register i%4
+  // 32x32 FP32-output MFMAs: D[i][j] in lane j + 32 ((i / 4) % 2), register
+  // 4 (i / 8) + i % 4 (measured on gfx950)
   float *Gc = P.G + ((int64_t)cl * P.slots + w * (int64_t)nb * nb * nb) * P.W3;
 #pragma unroll
   for (int g = 0; g < FN; g++) {
@@ -232,11 +246,11 @@ __global__ __launch_bounds__(256) void doubles_tri_kernel(TriParams P) {
 #pragma unroll
     for (int f = 0; f < FM; f++) {
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int64_t m =
-            (int64_t)mt * MT + wm * (MT / 2) + f * 16 + 4 * kk + r;
+      for (int r = 0; r < 16; r++) {
+        const int64_t m = (int64_t)mt * MT + wm * (MT / 2) + f * 32 +
+                          8 * (r >> 2) + 4 * kh + (r & 3);
         if (m >= N2) continue;
-        Gs[m * No + rr] = acc[f][g][r];  // GEMM order: 64-byte rows
+        Gs[m * No + rr] = acc[f][g][r];  // GEMM order: 128-byte rows
       }
     }
   }
@@ -284,15 +298,15 @@ hipError_t launch_doubles_tri(const DevSystem &s, const BatchArgs &b,
   P.Kv = s.Kv;
   P.Ko = s.Ko;
   P.N2 = s.No * s.No;
-  P.N2p = pad16(P.N2);
+  P.N2p = pad32(P.N2);
   P.W3 = s.No * s.No * s.No;
   P.slots = b.slots;
   P.first = b.first;
   P.last = b.last;
   P.nb = b.nb;
-  tile = pick_tile(tile, s.No, b.nb);
-  const int FM = tile_fm(tile), FN = tile_fn(tile);
-  const int64_t MT = 32 * FM, NT = 32 * FN;
+  tile = pick_tri_tile(tile, s.No, b.nb);
+  const int FM = tri_fm(tile), FN = tri_fn(tile);
+  const int64_t MT = 64 * FM, NT = 64 * FN;
   P.mtiles = (int)((P.N2 + MT - 1) / MT);
   P.ntiles = (int)(((int64_t)b.nb * s.No + NT - 1) / NT);
   P.total = (int64_t)b.nw * 3 * b.nb * b.nb * P.mtiles * P.ntiles;
@@ -303,8 +317,8 @@ hipError_t launch_doubles_tri(const DevSystem &s, const BatchArgs &b,
   if (FM == A && FN == B)                                                    \
     hipLaunchKernelGGL((doubles_tri_kernel<A, B>), dim3((unsigned)grid),     \
                        dim3(256), 0, st, P);
-  LAUNCH(2, 2) LAUNCH(4, 2) LAUNCH(2, 4) LAUNCH(4, 4) LAUNCH(3, 3)
-  LAUNCH(5, 5) LAUNCH(4, 5) LAUNCH(5, 4)
+  LAUNCH(2, 2) LAUNCH(2, 3) LAUNCH(3, 2) LAUNCH(2, 4) LAUNCH(4, 2)
+  LAUNCH(3, 3) LAUNCH(2, 5) LAUNCH(5, 2)
 #undef LAUNCH
   return hipGetLastError();
 }

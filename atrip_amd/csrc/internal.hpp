@@ -19,16 +19,16 @@ constexpr int KCHUNK = 16;
 __host__ __device__ inline int64_t pad16(int64_t n) { return (n + 15) / 16 * 16; }
 
 // bf16x3 operands.  An FP64 operand [o][r < nr][k < K] is held as three
-// bf16 parts per element (x ~ b0 + b1 + b2) in 16 x 16 tiles of (R, k),
-// R = o * nrp + r (nrp: nr, or nr padded to 16 so that every o starts a
-// tile row).  Tile (R / 16, k / 16) sits at ((R / 16) (K / 16) + k / 16)
-// * 768 parts.  With L = R % 16 + 16 ((k % 16) / 4), the lane that feeds
-// (R, k) to the MFMAs, and s = k % 4, parts 0 and 1 sit at 8 L + s and
-// 8 L + 4 + s (16 bytes per lane: [b0 x4, b1 x4]) and part 2 at
-// 512 + 4 L + s, so a 16 x 16 fragment chunk is one contiguous 1 KiB of
-// parts 0/1 and one contiguous 512 bytes of part 2 in lane order.
-// The A operands (nr = No^2 rows (p, q)) also keep a row-swapped copy whose
-// row p + No q holds row q + No p of the straight one.
+// bf16 parts per element (x ~ b0 + b1 + b2) in tiles of 32 rows x 16 k,
+// R = o * nrp + r (nrp: nr, or nr padded to 32 so that every o starts a
+// tile row).  Tile (R / 32, k / 16) sits at ((R / 32) (K / 16) + k / 16)
+// * 1536 parts; inside it part p of (R, k) sits at 512 p + 8 L + k % 8 with
+// L = R % 32 + 32 ((k % 16) / 8), the lane that feeds (R, k) to a 32x32x16
+// MFMA, so each part of a 32 x 16 fragment chunk is 1 KiB contiguous in
+// lane order (one 16-byte load per lane).  The A operands (nr = No^2 rows
+// (p, q)) also keep a row-swapped copy whose row p + No q holds row
+// q + No p.
+__host__ __device__ inline int64_t pad32(int64_t n) { return (n + 31) / 32 * 32; }
 struct TriGeom {
   int64_t no, nr, nrp, K;
   bool swapped;  // keeps a row-swapped copy (A operands)
@@ -37,16 +37,14 @@ struct TriOp {
   uint16_t *p = nullptr, *sw = nullptr;
 };
 __host__ __device__ inline int64_t tri_parts(const TriGeom &g) {
-  return 3 * pad16(g.no * g.nrp) * g.K;
+  return 3 * pad32(g.no * g.nrp) * g.K;
 }
-// Offsets (in parts) of part 0 (part 1 four further) and part 2 of (R, k).
-struct TriPos {
-  int64_t p01, p2;
-};
-__host__ __device__ inline TriPos tri_pos(int64_t R, int64_t k, int64_t K) {
-  const int64_t t = ((R >> 4) * (K >> 4) + (k >> 4)) * 768;
-  const int64_t L = (R & 15) + 16 * ((k & 15) >> 2);
-  return TriPos{t + 8 * L + (k & 3), t + 512 + 4 * L + (k & 3)};
+// Start of tile (R / 32, k / 16) and the lane slot of (R, k).
+__host__ __device__ inline int64_t tri_tile(int64_t R, int64_t k, int64_t K) {
+  return ((R >> 5) * (K >> 4) + (k >> 4)) * 1536;
+}
+__host__ __device__ inline int tri_lane(int64_t R, int64_t k) {
+  return (int)((R & 31) + 32 * ((k & 15) >> 3));
 }
 
 // Layout of the per-tuple class buffers written by the doubles kernels and
@@ -104,7 +102,7 @@ struct DevSystem {
   Split sT3, sV3, sH3, sT2N, sT2TN, sV3J, sH3J;
   bool split_ready = false;
   // bf16x3 copies in the tiled layout (TriGeom): T3 and H3 rows padded to
-  // 16 per slice with row-swapped copies, the pair-indexed arrays unpadded.
+  // 32 per slice with row-swapped copies, the pair-indexed arrays unpadded.
   TriOp tT3, tV3, tH3, tT2N, tT2TN, tV3J, tH3J;
   bool tri_ready = false;
   // Operand store: 0 FP64 (split or bf16x3 copies made on the first mixed
@@ -136,17 +134,19 @@ __device__ inline void split_store(double x, float *h, uint16_t *l,
   if (l) l[q] = to_bf16((float)(x - (double)hi));
 }
 
-// bf16x3 parts of x at position `at` of a tiled copy: b0 = bf16(x),
+// bf16x3 parts of x into element (R, k) of a tiled copy: b0 = bf16(x),
 // b1 = bf16(x - b0), b2 = bf16(x - b0 - b1), each residual formed in FP64,
 // so b0 + b1 + b2 is x within 2^-27 |x|.
-__device__ inline void tri_put(double x, uint16_t *out, const TriPos &at) {
+__device__ inline void tri_put(double x, uint16_t *out, int64_t R, int64_t k,
+                               int64_t K) {
   const uint16_t p0 = to_bf16((float)x);
   const double r1 = x - (double)__uint_as_float((uint32_t)p0 << 16);
   const uint16_t p1 = to_bf16((float)r1);
   const double r2 = r1 - (double)__uint_as_float((uint32_t)p1 << 16);
-  out[at.p01] = p0;
-  out[at.p01 + 4] = p1;
-  out[at.p2] = to_bf16((float)r2);
+  const int64_t at = tri_tile(R, k, K) + 8 * tri_lane(R, k) + (k & 7);
+  out[at] = p0;
+  out[at + 512] = p1;
+  out[at + 1024] = to_bf16((float)r2);
 }
 
 // Element q of FP64 operand [o][r][k] into its tiled copy (and the
@@ -154,10 +154,10 @@ __device__ inline void tri_put(double x, uint16_t *out, const TriPos &at) {
 __device__ inline void tri_store(double x, const TriOp &t, const TriGeom &g,
                                  int64_t No, int64_t q) {
   const int64_t k = q % g.K, row = q / g.K, r = row % g.nr, o = row / g.nr;
-  tri_put(x, t.p, tri_pos(o * g.nrp + r, k, g.K));
+  tri_put(x, t.p, o * g.nrp + r, k, g.K);
   if (g.swapped) {
     const int64_t rs = r / No + No * (r % No);
-    tri_put(x, t.sw, tri_pos(o * g.nrp + rs, k, g.K));
+    tri_put(x, t.sw, o * g.nrp + rs, k, g.K);
   }
 }
 
@@ -305,7 +305,7 @@ hipError_t launch_tri(const double *x, const TriOp &out, const TriGeom &g,
 // Geometry of a system's bf16x3 operands: A (T3, H3 and H3J) and the
 // pair-indexed B arrays (V3, T2N, T2TN, V3J), for contraction length K.
 inline TriGeom tri_geom_a(int64_t No, int64_t Nv, int64_t K) {
-  return TriGeom{Nv, No * No, pad16(No * No), K, true};
+  return TriGeom{Nv, No * No, pad32(No * No), K, true};
 }
 inline TriGeom tri_geom_b(int64_t No, int64_t Nv, int64_t K) {
   return TriGeom{Nv * Nv, No, No, K, false};
