@@ -58,11 +58,11 @@ struct TriParams {
 // variant t of the "tile" option.
 constexpr int N_TRI_TILES = 8;
 __host__ __device__ inline int tri_fm(int t) {
-  const int v[N_TRI_TILES] = {2, 2, 3, 2, 4, 3, 2, 5};
+  const int v[N_TRI_TILES] = {2, 2, 3, 2, 3, 2, 3, 4};
   return v[t];
 }
 __host__ __device__ inline int tri_fn(int t) {
-  const int v[N_TRI_TILES] = {2, 3, 2, 4, 2, 3, 5, 2};
+  const int v[N_TRI_TILES] = {2, 3, 2, 4, 3, 5, 5, 4};
   return v[t];
 }
 // Relative rate of each variant (operand bytes per MFMA cycle fall with
@@ -204,29 +204,25 @@ __global__ __launch_bounds__(256) void doubles_tri_kernel(TriParams P) {
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[f][g][r] = 0.f;
 
-  // three operand sets in rotation: two chunks of loads stay in flight
-  // behind the MFMAs of the current one.  A chunk is 3 (FM + FN) loads, so
-  // three chunks fit the 63 of the vector-memory counter and the wait
-  // before a chunk's MFMAs covers that chunk's loads only.
-  static_assert(9 * (FM + FN) <= 63, "three chunks of loads must fit vmcnt");
-  Parts a0[FM], b0[FN], a1[FM], b1[FN], a2[FM], b2[FN];
+  // two operand sets in ping-pong: the loads of chunk c + 1 are in flight
+  // behind the MFMAs of chunk c (a chunk is 6 FM FN MFMAs of 32 cycles, at
+  // least 24 x 32 cycles; a third set would cost the registers the largest
+  // tiles need and measured slower)
+  static_assert(6 * (FM + FN) <= 63, "two chunks of loads must fit vmcnt");
+  Parts a0[FM], b0[FN], a1[FM], b1[FN];
 #define CLAMP(K_) ((K_) < last ? (K_) : last)
 #pragma unroll 1
   for (int sg = 0; sg < 4; sg++) {
     SET_PTRS(sg);
     const int64_t last = (sg < 2 ? Kv : Ko) - KCHUNK;
     LOAD(a0, b0, 0);
-    LOAD(a1, b1, CLAMP(KCHUNK));
-    for (int64_t k0 = 0;; k0 += 3 * KCHUNK) {
-      LOAD(a2, b2, CLAMP(k0 + 2 * KCHUNK));
+    for (int64_t k0 = 0;; k0 += 2 * KCHUNK) {
+      LOAD(a1, b1, CLAMP(k0 + KCHUNK));
       COMPUTE(a0, b0);
       if (k0 + KCHUNK > last) break;
-      LOAD(a0, b0, CLAMP(k0 + 3 * KCHUNK));
+      LOAD(a0, b0, CLAMP(k0 + 2 * KCHUNK));
       COMPUTE(a1, b1);
       if (k0 + 2 * KCHUNK > last) break;
-      LOAD(a1, b1, CLAMP(k0 + 4 * KCHUNK));
-      COMPUTE(a2, b2);
-      if (k0 + 3 * KCHUNK > last) break;
     }
   }
 #undef CLAMP
@@ -317,8 +313,8 @@ hipError_t launch_doubles_tri(const DevSystem &s, const BatchArgs &b,
   if (FM == A && FN == B)                                                    \
     hipLaunchKernelGGL((doubles_tri_kernel<A, B>), dim3((unsigned)grid),     \
                        dim3(256), 0, st, P);
-  LAUNCH(2, 2) LAUNCH(2, 3) LAUNCH(3, 2) LAUNCH(2, 4) LAUNCH(4, 2)
-  LAUNCH(3, 3) LAUNCH(2, 5) LAUNCH(5, 2)
+  LAUNCH(2, 2) LAUNCH(2, 3) LAUNCH(3, 2) LAUNCH(2, 4) LAUNCH(3, 3)
+  LAUNCH(2, 5) LAUNCH(3, 5) LAUNCH(4, 4)
 #undef LAUNCH
   return hipGetLastError();
 }
