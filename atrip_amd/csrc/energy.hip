@@ -128,6 +128,31 @@ __global__ __launch_bounds__(256) void wsum_kernel(const GT *A, const GT *B,
 // hold W at the six index permutations of those points are copied into LDS
 // with coalesced 4 KiB reads, then every point reads its twelve W/Z values
 // from LDS.
+// The orbit of block triple (B1, B2, B3) under the six index permutations
+// (ijk ikj jik jki kij kji): base offsets cb[s] of its nc distinct cubes and
+// the slot al[p] of the cube of each permutation.
+__device__ __forceinline__ void orbit(int B1, int B2, int B3, int nbk,
+                                      int (&cb)[6], int &nc, int (&al)[6]) {
+  const int cx[6] = {B1, B1, B2, B2, B3, B3};
+  const int cy[6] = {B2, B3, B1, B3, B1, B2};
+  const int cz[6] = {B3, B2, B3, B1, B2, B1};
+  nc = 0;
+#pragma unroll
+  for (int p = 0; p < 6; p++) {
+    int q0 = p;
+#pragma unroll
+    for (int q = 0; q < p; q++)
+      if (q0 == p && cx[q] == cx[p] && cy[q] == cy[p] && cz[q] == cz[p]) q0 = q;
+    if (q0 == p) {
+      al[p] = nc;
+      cb[nc < 6 ? nc : 5] = ((cz[p] * nbk + cy[p]) * nbk + cx[p]) << 9;
+      nc++;
+    } else {
+      al[p] = al[q0];
+    }
+  }
+}
+
 template <bool HJ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void energy_kernel(EnergyParams P) {
@@ -157,6 +182,7 @@ void energy_kernel(EnergyParams P) {
   __shared__ double cube[6][512];
   __shared__ double cubeJ[HJ ? 6 : 1][HJ ? 512 : 1];
   __shared__ double tA[64], tB[64], tC[64];  // No <= 64 handled in tiles below
+  __shared__ double sE[24];                  // eps_i at the triple's blocks
   // V^bc, V^ac, V^ab on the 8x8 block pairs (r, s) of the current triple
   // (positions r != s of blocks B1, B2, B3): sV[slice][r][s][ly + 8 lz]
   __shared__ double sV[3][3][3][64];
@@ -177,112 +203,114 @@ void energy_kernel(EnergyParams P) {
   const int tid = threadIdx.x;
 
   double e_acc = 0.0, ect_acc = 0.0;
+  // Block triples B1 >= B2 >= B3 in nested order (B1 fastest).  The loads
+  // of the next triple (cubes, V blocks, t and eps_i tables) are issued into
+  // registers before the points of the current one are computed and written
+  // to LDS after them, so each triple's memory latency hides behind the
+  // previous triple's arithmetic.  Loads are unconditional (the last triple
+  // fetches itself again) so none waits behind a branch, and the point loop
+  // reads LDS only.
+  double tt[12];             // t at (a,x), (b,x), (c,x) and eps_i, x in B1,B2,B3
+  double vv[7];              // V^bc, V^ac, V^ab block pairs
+  double vw[12];             // cubes of the orbit (2 elements per cube)
+  double vwJ[HJ ? 12 : 1];
+  int ncur = 0, nnext = 0;
+#define FETCH(F1, F2, F3, NC)                                                  \
+  {                                                                            \
+    const int t_ = tid & 7;                                                    \
+    const int xi_ = (F1) * 8 + t_, xj_ = (F2) * 8 + t_, xk_ = (F3) * 8 + t_;   \
+    const int ci_ = xi_ < no ? xi_ : 0, cj_ = xj_ < no ? xj_ : 0,              \
+              ck_ = xk_ < no ? xk_ : 0;                                        \
+    tt[0] = P.Tai[a + Nv * ci_]; tt[1] = P.Tai[a + Nv * cj_];                  \
+    tt[2] = P.Tai[a + Nv * ck_]; tt[3] = P.Tai[b + Nv * ci_];                  \
+    tt[4] = P.Tai[b + Nv * cj_]; tt[5] = P.Tai[b + Nv * ck_];                  \
+    tt[6] = P.Tai[c + Nv * ci_]; tt[7] = P.Tai[c + Nv * cj_];                  \
+    tt[8] = P.Tai[c + Nv * ck_]; tt[9] = P.ei[ci_];                            \
+    tt[10] = P.ei[cj_]; tt[11] = P.ei[ck_];                                    \
+    const int Bp_[3] = {(F1), (F2), (F3)};                                     \
+    _Pragma("unroll") for (int it = 0; it < 7; it++) {                         \
+      const int e = tid + 256 * it;                                            \
+      const int v = e / 576, rs = (e / 64) % 9, l = e & 63;                    \
+      const int r = rs / 3, sp = rs % 3;                                       \
+      const int y = Bp_[r] * 8 + (l & 7), z = Bp_[sp] * 8 + (l >> 3);          \
+      const double *V = v == 0 ? Vbc : v == 1 ? Vac : Vab;                     \
+      const bool ok_ = e < 3 * 9 * 64 && r != sp && y < No && z < No;          \
+      const double val_ = V[ok_ ? y + No * z : 0];                             \
+      vv[it] = ok_ ? val_ : 0.0;                                               \
+    }                                                                          \
+    int cb_[6], al_[6];                                                        \
+    orbit((F1), (F2), (F3), nbk, cb_, NC, al_);                                \
+    _Pragma("unroll") for (int it = 0; it < 12; it++) {                        \
+      const int p_ = it >> 1, q_ = tid + 256 * (it & 1);                       \
+      vw[it] = Ws[cb_[p_ < (NC) ? p_ : 0] + q_];                               \
+      if constexpr (HJ) vwJ[it] = Js[cb_[p_ < (NC) ? p_ : 0] + q_];            \
+    }                                                                          \
+  }
+#define COMMIT(C1, C2, C3, NC)                                                 \
+  {                                                                            \
+    if (tid < 8) {                                                             \
+      const int ii_ = (C1) * 8 + tid, jj_ = (C2) * 8 + tid,                    \
+                kk_ = (C3) * 8 + tid;                                          \
+      tA[tid] = ii_ < No ? P.tai_sign * tt[0] : 0.0;                           \
+      tA[8 + tid] = jj_ < No ? P.tai_sign * tt[1] : 0.0;                       \
+      tA[16 + tid] = kk_ < No ? P.tai_sign * tt[2] : 0.0;                      \
+      tB[tid] = ii_ < No ? P.tai_sign * tt[3] : 0.0;                           \
+      tB[8 + tid] = jj_ < No ? P.tai_sign * tt[4] : 0.0;                       \
+      tB[16 + tid] = kk_ < No ? P.tai_sign * tt[5] : 0.0;                      \
+      tC[tid] = ii_ < No ? P.tai_sign * tt[6] : 0.0;                           \
+      tC[8 + tid] = jj_ < No ? P.tai_sign * tt[7] : 0.0;                       \
+      tC[16 + tid] = kk_ < No ? P.tai_sign * tt[8] : 0.0;                      \
+      sE[tid] = tt[9];                                                         \
+      sE[8 + tid] = tt[10];                                                    \
+      sE[16 + tid] = tt[11];                                                   \
+    }                                                                          \
+    _Pragma("unroll") for (int it = 0; it < 7; it++) {                         \
+      const int e = tid + 256 * it;                                            \
+      if (e < 3 * 9 * 64) (&sV[0][0][0][0])[e] = vv[it];                       \
+    }                                                                          \
+    _Pragma("unroll") for (int it = 0; it < 12; it++) {                        \
+      const int p_ = it >> 1, q_ = tid + 256 * (it & 1);                       \
+      if (p_ < (NC)) {                                                         \
+        cube[p_][q_] = vw[it];                                                 \
+        if constexpr (HJ) cubeJ[p_][q_] = vwJ[it];                             \
+      }                                                                        \
+    }                                                                          \
+  }
+  int B1 = 0, B2 = 0, B3 = 0;
+  FETCH(0, 0, 0, ncur);
 #pragma unroll 1
-  for (int B3 = 0; B3 < nbk; B3++)
-#pragma unroll 1
-    for (int B2 = B3; B2 < nbk; B2++)
-#pragma unroll 1
-      for (int B1 = B2; B1 < nbk; B1++) {
-        // cube coordinates of the permutations ijk ikj jik jki kij kji
-        const int cx[6] = {B1, B1, B2, B2, B3, B3};
-        const int cy[6] = {B2, B3, B1, B3, B1, B2};
-        const int cz[6] = {B3, B2, B3, B1, B2, B1};
-        int alias[6];
-#pragma unroll
-        for (int p = 0; p < 6; p++) {
-          alias[p] = p;
-          for (int q = 0; q < p; q++)
-            if (alias[p] == p && cx[q] == cx[p] && cy[q] == cy[p] &&
-                cz[q] == cz[p])
-              alias[p] = q;
-        }
-        __syncthreads();  // previous triple done with the LDS cubes
-        // singles tables for the three blocks (indices i,j,k < No)
-        if (tid < 8) {
-          const int i = B1 * 8 + tid, j = B2 * 8 + tid, k = B3 * 8 + tid;
-          // tA/tB/tC hold t at (a,x),(b,x),(c,x) for x in each block
-          tA[tid] = i < No ? P.tai_sign * P.Tai[a + Nv * i] : 0.0;
-          tA[8 + tid] = j < No ? P.tai_sign * P.Tai[a + Nv * j] : 0.0;
-          tA[16 + tid] = k < No ? P.tai_sign * P.Tai[a + Nv * k] : 0.0;
-          tB[tid] = i < No ? P.tai_sign * P.Tai[b + Nv * i] : 0.0;
-          tB[8 + tid] = j < No ? P.tai_sign * P.Tai[b + Nv * j] : 0.0;
-          tB[16 + tid] = k < No ? P.tai_sign * P.Tai[b + Nv * k] : 0.0;
-          tC[tid] = i < No ? P.tai_sign * P.Tai[c + Nv * i] : 0.0;
-          tC[8 + tid] = j < No ? P.tai_sign * P.Tai[c + Nv * j] : 0.0;
-          tC[16 + tid] = k < No ? P.tai_sign * P.Tai[c + Nv * k] : 0.0;
-        }
-        // doubles-integral blocks of the singles terms for this triple
-        {
-          const int Bp[3] = {B1, B2, B3};
-          double vv[7];
-#pragma unroll
-          for (int it = 0; it < 7; it++) {
-            const int e = tid + 256 * it;  // < 3 * 9 * 64
-            vv[it] = 0.0;
-            if (e < 3 * 9 * 64) {
-              const int v = e / 576, rs = (e / 64) % 9, l = e & 63;
-              const int r = rs / 3, sp = rs % 3;
-              const int y = Bp[r] * 8 + (l & 7), z = Bp[sp] * 8 + (l >> 3);
-              const double *V = v == 0 ? Vbc : v == 1 ? Vac : Vab;
-              if (r != sp && y < No && z < No) vv[it] = V[y + No * z];
-            }
-          }
-#pragma unroll
-          for (int it = 0; it < 7; it++) {
-            const int e = tid + 256 * it;
-            if (e < 3 * 9 * 64) (&sV[0][0][0][0])[e] = vv[it];
-          }
-        }
-        // distinct cubes of this orbit, loaded as one flat coalesced sweep
-        int cbase[6];
-        int nc = 0, slotof[6];
-#pragma unroll
-        for (int p = 0; p < 6; p++) {
-          slotof[p] = p;
-          if (alias[p] == p) {
-            cbase[nc] = ((cz[p] * nbk + cy[p]) * nbk + cx[p]) << 9;
-            slotof[p] = nc++;
-          }
-        }
-#pragma unroll
-        for (int p = 0; p < 6; p++) alias[p] = slotof[alias[p]];
-        // every load of this thread is issued before the first is used
-        // (2 elements per cube per thread), so the cube fetch costs one
-        // memory latency instead of one per element
-        {
-          double vw[12];
-          // loads are unconditional (a valid cube when p >= nc) so none
-          // waits behind a branch
-#pragma unroll
-          for (int it = 0; it < 12; it++) {
-            const int p = it >> 1, q = tid + 256 * (it & 1);
-            vw[it] = Ws[cbase[p < nc ? p : 0] + q];
-          }
-#pragma unroll
-          for (int it = 0; it < 12; it++) {
-            const int p = it >> 1, q = tid + 256 * (it & 1);
-            if (p < nc) cube[p][q] = vw[it];
-          }
-          if constexpr (HJ) {
-#pragma unroll
-            for (int it = 0; it < 12; it++) {
-              const int p = it >> 1, q = tid + 256 * (it & 1);
-              vw[it] = Js[cbase[p < nc ? p : 0] + q];
-            }
-#pragma unroll
-            for (int it = 0; it < 12; it++) {
-              const int p = it >> 1, q = tid + 256 * (it & 1);
-              if (p < nc) cubeJ[p][q] = vw[it];
-            }
-          }
-        }
-        __syncthreads();
+  for (;;) {
+    __syncthreads();  // the previous triple's points are done with the LDS
+    COMMIT(B1, B2, B3, ncur);
+    __syncthreads();
+    int N1 = B1 + 1, N2b = B2, N3 = B3;
+    if (N1 == nbk) {
+      N2b++;
+      if (N2b == nbk) {
+        N3++;
+        N2b = N3;
+      }
+      N1 = N2b;
+    }
+    const bool more = N3 < nbk;
+    if (!more) {
+      N1 = B1;
+      N2b = B2;
+      N3 = B3;
+    }
+    FETCH(N1, N2b, N3, nnext);
+    int alias[6];
+    {
+      int cb_[6], nc_;
+      orbit(B1, B2, B3, nbk, cb_, nc_, alias);
+    }
+    {
 #pragma unroll 1
         for (int pt = tid; pt < 512; pt += 256) {
           const int li = pt & 7, lj = (pt >> 3) & 7, lk = pt >> 6;
           const int i = B1 * 8 + li, j = B2 * 8 + lj, k = B3 * 8 + lk;
           if (i >= no || j >= no || k >= no || i < j || j < k) continue;
-          const double ei = P.ei[i], ej = P.ei[j], ek = P.ei[k];
+          const double ei = sE[li], ej = sE[8 + lj], ek = sE[16 + lk];
           const double facjk = j == k ? 0.5 : 1.0;
           const double facij = i == j ? 0.5 : 1.0;
           const double eijk = (ei + ej) + ek;
@@ -350,7 +378,15 @@ void energy_kernel(EnergyParams P) {
             }
           }
         }
-      }
+    }
+    if (!more) break;
+    B1 = N1;
+    B2 = N2b;
+    B3 = N3;
+    ncur = nnext;
+  }
+#undef FETCH
+#undef COMMIT
 
   e_acc = wave_sum(e_acc);
   ect_acc = wave_sum(ect_acc);
