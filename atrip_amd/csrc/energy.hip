@@ -21,7 +21,7 @@ namespace atrip_amd {
 namespace {
 
 struct EnergyParams {
-  const void *W, *WJ;  // W (and the (cT) pass's W) as cubes, [slot][W3]
+  const double *W, *WJ;  // W (and the (cT) pass's W) as cubes, [slot][W3]
   const double *Tai, *VHH, *ei, *ea;
   const int32_t *windows;
   const uint64_t *aprefix;
@@ -40,13 +40,10 @@ __device__ inline double wave_sum(double v) {
 }
 
 // Class sums of one 16x16x16 brick of W for every slot of the batch:
-// out = sum(A) [+ sum(B)], sum(X) = (Xw + Xv) + Xu, summed in FP64 and
-// written as cubes of OT (FP64; FP32 for the FP32 class buffers of the
-// mixed forms, whose elements already carry FP32 accumulation errors far
-// above the one rounding of the stored sum).
-template <typename GT, typename OT>
+// out = sum(A) [+ sum(B)], sum(X) = (Xw + Xv) + Xu, written as cubes.
+template <typename GT>
 __global__ __launch_bounds__(256) void wsum_kernel(const GT *A, const GT *B,
-                                                   OT *out, int64_t No,
+                                                   double *out, int64_t No,
                                                    int64_t W3g, int64_t slots,
                                                    int nbk, int nbr) {
   // LDS brick [z][y][x] with pitch 17 (conflict-free transposed writes)
@@ -58,7 +55,7 @@ __global__ __launch_bounds__(256) void wsum_kernel(const GT *A, const GT *B,
   const int tid = threadIdx.x, no = (int)No;
   const int x0 = 16 * bx, y0 = 16 * by, z0 = 16 * bz;
   auto at = [](int x, int y, int z) { return x + 17 * (y + 17 * z); };
-  OT *o = out + slot * (int64_t)(512 * nbk * nbk * nbk);
+  double *o = out + slot * (int64_t)(512 * nbk * nbk * nbk);
   for (int set = 0; set < (B ? 2 : 1); set++) {
     const GT *S = set == 0 ? A : B;
     const GT *Sw = S + slot * W3g, *Sv = S + (slots + slot) * W3g,
@@ -120,8 +117,8 @@ __global__ __launch_bounds__(256) void wsum_kernel(const GT *A, const GT *B,
       const int gx = 2 * bx + cx, gy = 2 * by + cy, gz = 2 * bz + cz;
       if (gx >= nbk || gy >= nbk || gz >= nbk) continue;
       const double val = br[at(8 * cx + (q & 7), 8 * cy + ((q >> 3) & 7), 8 * cz + (q >> 6))];
-      OT *dst = o + ((int64_t)((gz * nbk + gy) * nbk + gx) << 9) + q;
-      *dst = (OT)(set == 0 ? val : (double)*dst + val);
+      double *dst = o + ((int64_t)((gz * nbk + gy) * nbk + gx) << 9) + q;
+      *dst = set == 0 ? val : *dst + val;
     }
   }
 }
@@ -156,7 +153,7 @@ __device__ __forceinline__ void orbit(int B1, int B2, int B3, int nbk,
   }
 }
 
-template <bool HJ, typename WT>
+template <bool HJ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void energy_kernel(EnergyParams P) {
   const int64_t gslot = blockIdx.x;
@@ -194,8 +191,8 @@ void energy_kernel(EnergyParams P) {
   const double *Vbc = P.VHH + (b + Nv * c) * N2;
   const double *Vac = P.VHH + (a + Nv * c) * N2;
   const double *Vab = P.VHH + (a + Nv * b) * N2;
-  const WT *Ws = (const WT *)P.W + gslot * P.W3;
-  const WT *Js = have_j ? (const WT *)P.WJ + gslot * P.W3 : Ws;
+  const double *Ws = P.W + gslot * P.W3;
+  const double *Js = have_j ? P.WJ + gslot * P.W3 : Ws;
 
   int distinct = 0;
   if (a == b) distinct++;
@@ -215,8 +212,8 @@ void energy_kernel(EnergyParams P) {
   // reads LDS only.
   double tt[12];             // t at (a,x), (b,x), (c,x) and eps_i, x in B1,B2,B3
   double vv[7];              // V^bc, V^ac, V^ab block pairs
-  WT vw[12];                 // cubes of the orbit (2 elements per cube)
-  WT vwJ[HJ ? 12 : 1];
+  double vw[12];             // cubes of the orbit (2 elements per cube)
+  double vwJ[HJ ? 12 : 1];
   int ncur = 0, nnext = 0;
 #define FETCH(F1, F2, F3, NC)                                                  \
   {                                                                            \
@@ -274,8 +271,8 @@ void energy_kernel(EnergyParams P) {
     _Pragma("unroll") for (int it = 0; it < 12; it++) {                        \
       const int p_ = it >> 1, q_ = tid + 256 * (it & 1);                       \
       if (p_ < (NC)) {                                                         \
-        cube[p_][q_] = (double)vw[it];                                         \
-        if constexpr (HJ) cubeJ[p_][q_] = (double)vwJ[it];                     \
+        cube[p_][q_] = vw[it];                                                 \
+        if constexpr (HJ) cubeJ[p_][q_] = vwJ[it];                             \
       }                                                                        \
     }                                                                          \
   }
@@ -452,19 +449,19 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(
 }  // namespace
 
 hipError_t launch_wsum(const DevSystem &s, const BatchArgs &b, const void *A,
-                       const void *B, bool g32, void *out, hipStream_t st) {
+                       const void *B, bool g32, double *out, hipStream_t st) {
   const int nbk = (int)w_blocks(s.No), nbr = (nbk + 1) / 2;
   const int64_t grid = b.slots * (int64_t)nbr * nbr * nbr;
   if (grid <= 0) return hipSuccess;
   const int64_t W3g = s.No * s.No * s.No;
   if (g32)
-    hipLaunchKernelGGL((wsum_kernel<float, float>), dim3((unsigned)grid),
-                       dim3(256), 0, st, (const float *)A, (const float *)B,
-                       (float *)out, s.No, W3g, b.slots, nbk, nbr);
+    hipLaunchKernelGGL(wsum_kernel<float>, dim3((unsigned)grid), dim3(256), 0,
+                       st, (const float *)A, (const float *)B, out, s.No, W3g,
+                       b.slots, nbk, nbr);
   else
-    hipLaunchKernelGGL((wsum_kernel<double, double>), dim3((unsigned)grid),
-                       dim3(256), 0, st, (const double *)A, (const double *)B,
-                       (double *)out, s.No, W3g, b.slots, nbk, nbr);
+    hipLaunchKernelGGL(wsum_kernel<double>, dim3((unsigned)grid), dim3(256),
+                       0, st, (const double *)A, (const double *)B, out, s.No,
+                       W3g, b.slots, nbk, nbr);
   return hipGetLastError();
 }
 
@@ -472,10 +469,8 @@ hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
                          const void *G, const void *GJ, bool g32,
                          double tai_sign, double *e_slot, double *wcube,
                          hipStream_t st) {
-  // cubes of FP32 class sums for the mixed forms, FP64 otherwise
-  const int64_t W3c = w_cube(s.No), wsz = g32 ? 4 : 8;
-  void *W = wcube;
-  void *WJ = GJ ? (void *)((char *)wcube + b.slots * W3c * wsz) : nullptr;
+  const int64_t W3c = w_cube(s.No);
+  double *W = wcube, *WJ = GJ ? wcube + b.slots * W3c : nullptr;
   hipError_t e = launch_wsum(s, b, G, nullptr, g32, W, st);
   if (e != hipSuccess) return e;
   if (GJ && (e = launch_wsum(s, b, GJ, nullptr, g32, WJ, st)) != hipSuccess)
@@ -501,14 +496,10 @@ hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
   P.nb = b.nb;
   P.tai_sign = tai_sign;
   const dim3 grid((unsigned)b.slots), block(256);
-  if (GJ && g32)
-    hipLaunchKernelGGL((energy_kernel<true, float>), grid, block, 0, st, P);
-  else if (GJ)
-    hipLaunchKernelGGL((energy_kernel<true, double>), grid, block, 0, st, P);
-  else if (g32)
-    hipLaunchKernelGGL((energy_kernel<false, float>), grid, block, 0, st, P);
+  if (GJ)
+    hipLaunchKernelGGL(energy_kernel<true>, grid, block, 0, st, P);
   else
-    hipLaunchKernelGGL((energy_kernel<false, double>), grid, block, 0, st, P);
+    hipLaunchKernelGGL(energy_kernel<false>, grid, block, 0, st, P);
   return hipGetLastError();
 }
 

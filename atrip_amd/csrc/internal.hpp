@@ -256,10 +256,9 @@ hipError_t launch_doubles_tri(const DevSystem &s, const BatchArgs &b,
                               bool use_j, float *G, int tile, hipStream_t st);
 
 // Class sums: out[slot] = sum(A) (+ sum(B)) as cubes (w_cube per slot), for
-// class-buffer sets A, B: FP64 in and out, or FP32 in and out when g32
-// (summed in FP64 either way).
+// class-buffer sets A, B (FP64, or FP32 when g32).
 hipError_t launch_wsum(const DevSystem &s, const BatchArgs &b, const void *A,
-                       const void *B, bool g32, void *out, hipStream_t st);
+                       const void *B, bool g32, double *out, hipStream_t st);
 
 // Class sums and singles + energy epilogue; writes e_slot[2*slot + {0,1}].
 // wcube: scratch of (GJ ? 2 : 1) * slots * w_cube(No) doubles.
