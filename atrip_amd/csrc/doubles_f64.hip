@@ -152,13 +152,19 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
         load_chunk(bp[g] + (KOFF));                                     \
     __builtin_amdgcn_sched_barrier(0);                                  \
   }
-#define COMPUTE(AV, BV)                                                 \
+  // k-steps 0, 1 hold kappa k0 .. k0 + 7 and steps 2, 3 kappa k0 + 8 ..
+  // k0 + 15: a last chunk with at most 8 occupied kappa (the rest zero
+  // padding, which adds exact zeros) skips steps 2, 3
+#define STEPS(AV, BV, S0, S1)                                           \
+  _Pragma("unroll") for (int s = S0; s < S1; s++)                       \
+      _Pragma("unroll") for (int f = 0; f < FM; f++)                    \
+          _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =    \
+              __builtin_amdgcn_mfma_f64_16x16x4f64(AV[f][s], BV[g][s],  \
+                                                   acc[f][g], 0, 0, 0);
+#define COMPUTE(AV, BV, HALF)                                           \
   {                                                                     \
-    _Pragma("unroll") for (int s = 0; s < 4; s++)                       \
-        _Pragma("unroll") for (int f = 0; f < FM; f++)                  \
-            _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =  \
-                __builtin_amdgcn_mfma_f64_16x16x4f64(AV[f][s], BV[g][s], \
-                                                     acc[f][g], 0, 0, 0); \
+    STEPS(AV, BV, 0, 2)                                                 \
+    if (!(HALF)) { STEPS(AV, BV, 2, 4) }                                \
   }
   d4 a0[FM], b0[FN], a1[FM], b1[FN];
 #pragma unroll 1
@@ -167,17 +173,20 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
     // the prefetch is issued unconditionally (clamped to the last chunk) so
     // every path into a COMPUTE has the same loads outstanding
     const int64_t last = (sg < 2 ? Kv : Ko) - KCHUNK;
+    // the segment's last chunk holds at most 8 occupied kappa
+    const bool tail8 = (sg < 2 ? P.Nv : No) - last <= 8;
     LOAD(a0, b0, 0);
     for (int64_t k0 = 0;; k0 += 2 * KCHUNK) {
       LOAD(a1, b1, k0 + KCHUNK < last ? k0 + KCHUNK : last);
-      COMPUTE(a0, b0);
+      COMPUTE(a0, b0, tail8 && k0 == last);
       if (k0 + KCHUNK > last) break;
       LOAD(a0, b0, k0 + 2 * KCHUNK < last ? k0 + 2 * KCHUNK : last);
-      COMPUTE(a1, b1);
+      COMPUTE(a1, b1, tail8 && k0 + KCHUNK == last);
       if (k0 + 2 * KCHUNK > last) break;
     }
   }
 #undef LOAD
+#undef STEPS
 #undef COMPUTE
 
 #undef SET_PTRS
