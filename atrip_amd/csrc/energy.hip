@@ -179,8 +179,12 @@ void energy_kernel(EnergyParams P) {
   }
   constexpr bool have_j = HJ;
 
-  __shared__ double cube[6][512];
-  __shared__ double cubeJ[HJ ? 6 : 1][HJ ? 512 : 1];
+  // cube element x + 8 y + 64 z at x + 8 y + CZ z: the six permutation
+  // reads of a wave (lanes over two of x, y, z) are then at most 2-way bank
+  // conflicts instead of 8-way
+  constexpr int CZ = 69, CSZ = 7 * CZ + 64;
+  __shared__ double cube[6][CSZ];
+  __shared__ double cubeJ[HJ ? 6 : 1][HJ ? CSZ : 1];
   __shared__ double tA[64], tB[64], tC[64];  // No <= 64 handled in tiles below
   __shared__ double sE[24];                  // eps_i at the triple's blocks
   // V^bc, V^ac, V^ab on the 8x8 block pairs (r, s) of the current triple
@@ -271,8 +275,9 @@ void energy_kernel(EnergyParams P) {
     _Pragma("unroll") for (int it = 0; it < 12; it++) {                        \
       const int p_ = it >> 1, q_ = tid + 256 * (it & 1);                       \
       if (p_ < (NC)) {                                                         \
-        cube[p_][q_] = vw[it];                                                 \
-        if constexpr (HJ) cubeJ[p_][q_] = vwJ[it];                             \
+        const int at_ = (q_ & 63) + CZ * (q_ >> 6);                            \
+        cube[p_][at_] = vw[it];                                                \
+        if constexpr (HJ) cubeJ[p_][at_] = vwJ[it];                            \
       }                                                                        \
     }                                                                          \
   }
@@ -305,7 +310,9 @@ void energy_kernel(EnergyParams P) {
       orbit(B1, B2, B3, nbk, cb_, nc_, alias);
     }
     {
-#pragma unroll 1
+      // both points of a thread in one body, so their LDS reads and FP64
+      // chains interleave
+#pragma unroll
         for (int pt = tid; pt < 512; pt += 256) {
           const int li = pt & 7, lj = (pt >> 3) & 7, lk = pt >> 6;
           const int i = B1 * 8 + li, j = B2 * 8 + lj, k = B3 * 8 + lk;
@@ -316,9 +323,9 @@ void energy_kernel(EnergyParams P) {
           const double eijk = (ei + ej) + ek;
           const double denominator = epsabc - eijk;
           // local offsets of the permutations inside their cubes
-          const int o_ijk = li + 8 * lj + 64 * lk, o_ikj = li + 8 * lk + 64 * lj,
-                    o_jik = lj + 8 * li + 64 * lk, o_jki = lj + 8 * lk + 64 * li,
-                    o_kij = lk + 8 * li + 64 * lj, o_kji = lk + 8 * lj + 64 * li;
+          const int o_ijk = li + 8 * lj + CZ * lk, o_ikj = li + 8 * lk + CZ * lj,
+                    o_jik = lj + 8 * li + CZ * lk, o_jki = lj + 8 * lk + CZ * li,
+                    o_kij = lk + 8 * li + CZ * lj, o_kji = lk + 8 * lj + CZ * li;
           // singles at a permutation (x,y,z): t_a[x] V^bc_yz + t_b[y] V^ac_xz
           // + t_c[z] V^ab_xy; block slots: 0 -> i, 8 -> j, 16 -> k
           // slot s = 8 * position + local index (position 0: i, 1: j, 2: k)
