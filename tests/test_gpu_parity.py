@@ -221,6 +221,32 @@ def test_mixed_precision_cT_and_partition(hip_ctx, oracle, precision):
     assert np.array_equal(a, b)
 
 
+def test_mixed_absolute_error_at_molecular_scale(hip_ctx, oracle):
+    """The mixed forms' error is relative to |E(T)|: with the synthetic
+    amplitudes and integrals scaled down so that |E(T)| is of molecular size
+    (0.01-0.1 Eh), every mixed form is within 1e-7 Eh of FP64 (and FP64 of
+    the oracle).  The unscaled synthetic inputs give |E(T)| ~ 1e7-1e8 Eh,
+    where no FP64 or mixed result can be 1e-7 Eh absolute."""
+    no, nv, seed = 12, 36, 21
+    t = oracle.synthetic(no, nv, seed)
+    # T2 and V scaled by 0.1: E(T) = -0.092 Eh here
+    for name in ("Tabij", "Vabij", "Vijka", "Vabci"):
+        t[name] = t[name] * 0.1
+    e_ref, _ = oracle.run(no, nv, t)
+    print("molecular-scale E(T) oracle", e_ref, flush=True)
+    assert 1e-3 < abs(e_ref) < 1.0
+    s = hip_ctx.upload(no, nv, t)
+    try:
+        e64, _ = hip_ctx.run(s)
+        assert abs(e64 - e_ref) <= 1e-10 * abs(e_ref)
+        for precision in ("split_fp32", "fp32", "bf16x3"):
+            e, _ = hip_ctx.run(s, precision)
+            print(precision, "abs dE", abs(e - e64), flush=True)
+            assert abs(e - e64) <= 1e-7, (precision, e, e64)
+    finally:
+        hip_ctx.release(s)
+
+
 @pytest.mark.parametrize("precision", MIXED_FORMS)
 def test_mixed_tiles_and_windows(hip_ctx, oracle, precision):
     """Every tile variant and window shape gives the same per-tuple bits
