@@ -221,6 +221,24 @@ def test_mixed_precision_cT_and_partition(hip_ctx, oracle, precision):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("no,nv,seed", [(1, 5, 31), (2, 3, 32), (3, 2, 33), (33, 12, 34),
+                                        (65, 4, 35)])
+def test_mixed_edge_shapes(hip_ctx, oracle, no, nv, seed):
+    """Tiny and odd shapes through every mixed form: No = 1..3 (one padded
+    fragment), No = 33 (No^2 = 1089 rows, padded slices), No = 65 (> 64)."""
+    t, (e_ref, _, pt_ref, _) = _oracle_per_tuple(oracle, no, nv, seed)
+    s = hip_ctx.upload(no, nv, t)
+    try:
+        for precision in ("split_fp32", "fp32", "bf16x3"):
+            print(precision, no, nv, flush=True)
+            e, _ = hip_ctx.run(s, precision)
+            assert abs(e - e_ref) <= MIXED_REL_E * abs(e_ref) + 1e-15, (precision, e, e_ref)
+            pt = hip_ctx.tuple_energies(s, 0, oracle.n_tuples(nv), precision=precision)
+            assert np.max(np.abs(pt - pt_ref)) <= MIXED_REL_TUPLE * np.max(np.abs(pt_ref)) + 1e-18
+    finally:
+        hip_ctx.release(s)
+
+
 def test_mixed_absolute_error_at_molecular_scale(hip_ctx, oracle):
     """The mixed forms' error is relative to |E(T)|: with the synthetic
     amplitudes and integrals scaled down so that |E(T)| is of molecular size
