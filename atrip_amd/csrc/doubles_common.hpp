@@ -103,4 +103,42 @@ inline int pick_tile(int tile, int64_t No, int nb) {
   return pick;
 }
 
+// Wave tiles of FM x FN fragments of 32 x 32 (2 x 2 waves per workgroup)
+// of the 32x32-MFMA kernels (bf16x3, split FP32), variant t of the "tile"
+// option.
+constexpr int N_T32_TILES = 8;
+__host__ __device__ inline int t32_fm(int t) {
+  const int v[N_T32_TILES] = {2, 2, 3, 2, 3, 2, 3, 4};
+  return v[t];
+}
+__host__ __device__ inline int t32_fn(int t) {
+  const int v[N_T32_TILES] = {2, 3, 2, 4, 3, 5, 5, 4};
+  return v[t];
+}
+// Relative rate of each variant (operand bytes per MFMA cycle fall with
+// FM FN / (FM + FN)).
+inline double t32_rate(int t) {
+  const double fm = t32_fm(t), fn = t32_fn(t);
+  return fm * fn / (fm + fn);
+}
+// Automatic variant: rate times the useful fraction of the padded
+// M = No^2 by N = nb No tile grid.
+inline int pick_t32_tile(int tile, int64_t No, int nb) {
+  if (tile >= 0 && tile < N_T32_TILES) return tile;
+  double best = -1.0;
+  int pick = 0;
+  const double M = (double)No * No, N = (double)nb * No;
+  for (int t = 0; t < N_T32_TILES; t++) {
+    const double mt = 64.0 * t32_fm(t), nt = 64.0 * t32_fn(t);
+    const double use = M / (__builtin_ceil(M / mt) * mt) *
+                       N / (__builtin_ceil(N / nt) * nt);
+    const double score = t32_rate(t) * use;
+    if (score > best) {
+      best = score;
+      pick = t;
+    }
+  }
+  return pick;
+}
+
 }  // namespace atrip_amd

@@ -3,43 +3,37 @@
 // Same regrouped class GEMMs and tuple windows as the FP64 kernel
 // (doubles_f64.hip).  Every operand x is held as hi = (float)x plus the
 // residual lo = bf16(x - hi); a product is formed as
-//     hi_a hi_b                      on v_mfma_f32_16x16x4_f32
-//   + (hi_a lo_b + lo_a hi_b)        on one v_mfma_f32_16x16x32_bf16
-// (the FP32 form drops the bracket: one FP32 product per product)
-// with both MFMAs accumulating into the same FP32 accumulator.  For a K chunk
-// of 16, lane group q (lanes 16q..16q+15) holds kappa = k0 + 4q + s, s < 4,
-// of its row/column for the FP32 MFMAs; the bf16 instruction's 8 K slots of
-// the same lane group take [bf16(hi_a), lo_a] against [lo_b, bf16(hi_b)] at
-// those four kappa, so its K = 32 covers the 16 hi*lo and the 16 lo*hi
-// products of the chunk and bf16(hi) is made in registers from the hi
-// already loaded (only hi and lo are stored: 6 bytes per element).  The
-// correction terms carry the part of each product below FP32's 24 bits
-// (their own bf16 rounding is 2^-8 of an already 2^-24-small term).  The FP32 result of each
-// class GEMM is stored as FP32 (it carries no more bits) and widened to FP64
-// by the energy kernel; singles, the energy and all sums stay FP64
-// (energy.hip).
+//     hi_a hi_b                      on v_mfma_f32_32x32x2_f32
+//   + hi_a lo_b + lo_a hi_b          on two v_mfma_f32_32x32x16_bf16
+// (the FP32 form drops the corrections: one FP32 product per product), all
+// accumulating into the same FP32 accumulator.  Fragments are 32 rows
+// (columns); for a K chunk of 16, lane l feeds row/column l % 32 at the 8
+// consecutive kappa k0 + 8 (l / 32) + m: FP32 MFMA m (m < 8) takes kappa
+// m of each half, and the bf16 MFMAs take all 8 (bf16(hi) made in
+// registers from the hi already loaded, lo loaded): bf16(hi_a) . lo_b and
+// lo_a . bf16(hi_b).  Per 32 x 32 block and chunk: 8 x 64 + 2 x 32 cycles,
+// against 16 x 64 in FP64.  The corrections carry the part of each product
+// below FP32's 24 bits (their own bf16 rounding is 2^-8 of an already
+// 2^-24-small term).  The FP32 result of each class GEMM is stored as FP32
+// (it carries no more bits) and widened to FP64 by the class-sum kernel;
+// singles, the energy and all sums stay FP64 (energy.hip).
 #include "doubles_common.hpp"
 
 namespace atrip_amd {
 namespace {
 
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+typedef float f8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-// bf16 operands of the correction MFMA from a lane's four hi values h and
-// four lo values l (same kappa): A takes [bf16(h), l], B takes [l, bf16(h)].
-__device__ inline bf8 corr_a(f4 h, bf4 l) {
-  const bf4 b = __builtin_convertvector(h, bf4);
-  return __builtin_shufflevector(b, l, 0, 1, 2, 3, 4, 5, 6, 7);
-}
-__device__ inline bf8 corr_b(f4 h, bf4 l) {
-  const bf4 b = __builtin_convertvector(h, bf4);
-  return __builtin_shufflevector(l, b, 0, 1, 2, 3, 4, 5, 6, 7);
-}
+// A lane's 8 hi values and 8 lo values of a chunk.
+struct SParts {
+  f8 h;
+  bf8 l;
+};
 
 struct SplitParams {
-  // per operand family: hi (float) and lo (bf16)
+  // per operand family: hi (float) and lo (bf16), row-major [row][K]
   const float *T3h, *H3h, *V3h, *T2Nh, *T2TNh;
   const __bf16 *T3l, *H3l, *V3l, *T2Nl, *T2TNl;
   const int32_t *windows;
@@ -53,7 +47,7 @@ struct SplitParams {
 
 template <int FM, int FN, bool CORR>
 __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
-  constexpr int MT = 32 * FM, NT = 32 * FN;
+  constexpr int MT = 64 * FM, NT = 64 * FN;
   const int64_t bid = blockIdx.x;
   const int64_t logical = (bid % 8) * P.per_xcd + bid / 8;  // XCD-aware
   if (logical >= P.total) return;
@@ -69,7 +63,7 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
   const int o1 = pair / nb, o2 = pair % nb;
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int kk = lane >> 4, l16 = lane & 15;
+  const int kh = lane >> 5, l32 = lane & 31;
   const int wm = wave >> 1, wn = wave & 1;
   const int64_t No = P.No, N2 = P.N2, Nv = P.Nv;
 
@@ -78,7 +72,7 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
   bool any = false;
 #pragma unroll
   for (int g = 0; g < FN; g++) {
-    const int64_t n = (int64_t)nt * NT + wn * (NT / 2) + g * 16 + l16;
+    const int64_t n = (int64_t)nt * NT + wn * (NT / 2) + g * 32 + l32;
     const int t = (int)(n / No);
     crr[g] = (int32_t)(n % No);
     int64_t a, b, c, slot;
@@ -103,7 +97,7 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
   int32_t rowS[FM], rowW[FM];  // 32-bit: live all loop
 #pragma unroll
   for (int f = 0; f < FM; f++) {
-    int64_t m = (int64_t)mt * MT + wm * (MT / 2) + f * 16 + l16;
+    int64_t m = (int64_t)mt * MT + wm * (MT / 2) + f * 32 + l32;
     if (m >= N2) m = N2 - 1;
     const int64_t p = m % No, q = m / No;
     rowS[f] = (int32_t)m;
@@ -122,8 +116,8 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
     const __bf16 *Al_ = ((SG) < 2 ? P.T3l : P.H3l) + abase_;                  \
     _Pragma("unroll") for (int f = 0; f < FM; f++) {                          \
       const int64_t r_ = (((SG)&1) ? rowW[f] : rowS[f]) * K_;                 \
-      aph[f] = Ah_ + r_ + 4 * kk;                                             \
-      apc[f] = Al_ + r_ + 4 * kk;                                             \
+      aph[f] = Ah_ + r_ + 8 * kh;                                             \
+      apc[f] = Al_ + r_ + 8 * kh;                                             \
     }                                                                         \
     _Pragma("unroll") for (int g = 0; g < FN; g++) {                          \
       int64_t u_, v_;                                                         \
@@ -131,70 +125,73 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
       const int64_t c_ = ((u_ + Nv * v_) * No + crr[g]) * K_;                 \
       const float *Bh_ = wh_ == 0 ? P.V3h : wh_ == 1 ? P.T2Nh : P.T2TNh;      \
       const __bf16 *Bl_ = wh_ == 0 ? P.V3l : wh_ == 1 ? P.T2Nl : P.T2TNl;     \
-      bph[g] = Bh_ + c_ + 4 * kk;                                             \
-      bpc[g] = Bl_ + c_ + 4 * kk;                                             \
+      bph[g] = Bh_ + c_ + 8 * kh;                                             \
+      bpc[g] = Bl_ + c_ + 8 * kh;                                             \
     }                                                                         \
   }
-#define LOAD(AH, AL, BH, BL, KOFF)                                           \
+#define LOAD(A, B, KOFF)                                                     \
   {                                                                          \
     _Pragma("unroll") for (int f = 0; f < FM; f++) {                         \
-      AH[f] = *(const f4 *)(aph[f] + (KOFF));                                \
-      if constexpr (CORR) AL[f] = *(const bf4 *)(apc[f] + (KOFF));          \
+      A[f].h = *(const f8 *)(aph[f] + (KOFF));                               \
+      if constexpr (CORR) A[f].l = *(const bf8 *)(apc[f] + (KOFF));         \
     }                                                                        \
     _Pragma("unroll") for (int g = 0; g < FN; g++) {                         \
-      BH[g] = *(const f4 *)(bph[g] + (KOFF));                                \
-      if constexpr (CORR) BL[g] = *(const bf4 *)(bpc[g] + (KOFF));          \
+      B[g].h = *(const f8 *)(bph[g] + (KOFF));                               \
+      if constexpr (CORR) B[g].l = *(const bf8 *)(bpc[g] + (KOFF));         \
     }                                                                        \
     __builtin_amdgcn_sched_barrier(0);                                       \
   }
-#define COMPUTE(AH, AL, BH, BL)                                              \
+  // FP32 MFMA m over all blocks before m + 1 (consecutive MFMAs update
+  // different accumulators), then the two correction MFMAs
+#define COMPUTE(A, B)                                                        \
   {                                                                          \
-    _Pragma("unroll") for (int s = 0; s < 4; s++)                            \
+    _Pragma("unroll") for (int m = 0; m < 8; m++)                            \
         _Pragma("unroll") for (int f = 0; f < FM; f++)                       \
             _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =       \
-                __builtin_amdgcn_mfma_f32_16x16x4f32(AH[f][s], BH[g][s],     \
+                __builtin_amdgcn_mfma_f32_32x32x2f32(A[f].h[m], B[g].h[m],   \
                                                      acc[f][g], 0, 0, 0);    \
     if constexpr (CORR) {                                                    \
-      bf8 ca_[FM], cb_[FN];                                                  \
-      _Pragma("unroll") for (int f = 0; f < FM; f++) ca_[f] =                \
-          corr_a(AH[f], AL[f]);                                              \
-      _Pragma("unroll") for (int g = 0; g < FN; g++) cb_[g] =                \
-          corr_b(BH[g], BL[g]);                                              \
+      bf8 ha_[FM], hb_[FN];                                                  \
+      _Pragma("unroll") for (int f = 0; f < FM; f++) ha_[f] =                \
+          __builtin_convertvector(A[f].h, bf8);                              \
+      _Pragma("unroll") for (int g = 0; g < FN; g++) hb_[g] =                \
+          __builtin_convertvector(B[g].h, bf8);                              \
       _Pragma("unroll") for (int f = 0; f < FM; f++)                         \
           _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =         \
-              __builtin_amdgcn_mfma_f32_16x16x32_bf16(ca_[f], cb_[g],        \
+              __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha_[f], B[g].l,        \
+                                                      acc[f][g], 0, 0, 0);   \
+      _Pragma("unroll") for (int f = 0; f < FM; f++)                         \
+          _Pragma("unroll") for (int g = 0; g < FN; g++) acc[f][g] =         \
+              __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[f].l, hb_[g],        \
                                                       acc[f][g], 0, 0, 0);   \
     }                                                                        \
   }
 
-  f4 acc[FM][FN];
+  f16v acc[FM][FN];
 #pragma unroll
   for (int f = 0; f < FM; f++)
 #pragma unroll
-    for (int g = 0; g < FN; g++) acc[f][g] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int g = 0; g < FN; g++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[f][g][r] = 0.f;
 
-  // three operand sets in rotation: two chunks of loads stay in flight
-  // behind the MFMAs of the current one (FP32 MFMAs take half the cycles of
-  // FP64 ones, so one chunk of compute alone does not cover the latency)
-  f4 ah0[FM], bh0[FN], ah1[FM], bh1[FN], ah2[FM], bh2[FN];
-  bf4 ac0[FM], bc0[FN], ac1[FM], bc1[FN], ac2[FM], bc2[FN];
+  // two operand sets in ping-pong: the loads of chunk c + 1 are in flight
+  // behind the MFMAs of chunk c
+  static_assert(6 * (FM + FN) <= 63, "two chunks of loads must fit vmcnt");
+  SParts a0[FM], b0[FN], a1[FM], b1[FN];
 #define CLAMP(K_) ((K_) < last ? (K_) : last)
 #pragma unroll 1
   for (int sg = 0; sg < 4; sg++) {
     SET_PTRS(sg);
     const int64_t last = (sg < 2 ? Kv : Ko) - KCHUNK;
-    LOAD(ah0, ac0, bh0, bc0, 0);
-    LOAD(ah1, ac1, bh1, bc1, CLAMP(KCHUNK));
-    for (int64_t k0 = 0;; k0 += 3 * KCHUNK) {
-      LOAD(ah2, ac2, bh2, bc2, CLAMP(k0 + 2 * KCHUNK));
-      COMPUTE(ah0, ac0, bh0, bc0);
+    LOAD(a0, b0, 0);
+    for (int64_t k0 = 0;; k0 += 2 * KCHUNK) {
+      LOAD(a1, b1, CLAMP(k0 + KCHUNK));
+      COMPUTE(a0, b0);
       if (k0 + KCHUNK > last) break;
-      LOAD(ah0, ac0, bh0, bc0, CLAMP(k0 + 3 * KCHUNK));
-      COMPUTE(ah1, ac1, bh1, bc1);
+      LOAD(a0, b0, CLAMP(k0 + 2 * KCHUNK));
+      COMPUTE(a1, b1);
       if (k0 + 2 * KCHUNK > last) break;
-      LOAD(ah1, ac1, bh1, bc1, CLAMP(k0 + 4 * KCHUNK));
-      COMPUTE(ah2, ac2, bh2, bc2);
-      if (k0 + 3 * KCHUNK > last) break;
     }
   }
 #undef CLAMP
@@ -202,8 +199,8 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
 #undef LOAD
 #undef COMPUTE
 
-  // store: for the 16x16 FP32-output MFMAs, D[i][j] sits in lane
-  // j + 16*(i/4), register i%4 (measured on gfx950)
+  // 32x32 FP32-output MFMAs: D[i][j] in lane j + 32 ((i / 4) % 2), register
+  // 4 (i / 8) + i % 4
   float *Gc = P.G + ((int64_t)cl * P.slots + w * (int64_t)nb * nb * nb) * P.W3;
 #pragma unroll
   for (int g = 0; g < FN; g++) {
@@ -213,11 +210,11 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
 #pragma unroll
     for (int f = 0; f < FM; f++) {
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int64_t m =
-            (int64_t)mt * MT + wm * (MT / 2) + f * 16 + 4 * kk + r;
+      for (int r = 0; r < 16; r++) {
+        const int64_t m = (int64_t)mt * MT + wm * (MT / 2) + f * 32 +
+                          8 * (r >> 2) + 4 * kh + (r & 3);
         if (m >= N2) continue;
-        Gs[m * No + rr] = acc[f][g][r];  // GEMM order: 64-byte rows
+        Gs[m * No + rr] = acc[f][g][r];  // GEMM order: 128-byte rows
       }
     }
   }
@@ -269,9 +266,9 @@ hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
   P.first = b.first;
   P.last = b.last;
   P.nb = b.nb;
-  tile = pick_tile(tile, s.No, b.nb);
-  const int FM = tile_fm(tile), FN = tile_fn(tile);
-  const int64_t MT = 32 * FM, NT = 32 * FN;
+  tile = pick_t32_tile(tile, s.No, b.nb);
+  const int FM = t32_fm(tile), FN = t32_fn(tile);
+  const int64_t MT = 64 * FM, NT = 64 * FN;
   P.mtiles = (int)((P.N2 + MT - 1) / MT);
   P.ntiles = (int)(((int64_t)b.nb * s.No + NT - 1) / NT);
   P.total = (int64_t)b.nw * 3 * b.nb * b.nb * P.mtiles * P.ntiles;
@@ -287,8 +284,8 @@ hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
       hipLaunchKernelGGL((doubles_split_kernel<A, B, false>),                \
                          dim3((unsigned)grid), dim3(256), 0, st, P);         \
   }
-  LAUNCH(2, 2) LAUNCH(4, 2) LAUNCH(2, 4) LAUNCH(4, 4) LAUNCH(3, 3)
-  LAUNCH(5, 5) LAUNCH(4, 5) LAUNCH(5, 4)
+  LAUNCH(2, 2) LAUNCH(2, 3) LAUNCH(3, 2) LAUNCH(2, 4) LAUNCH(3, 3)
+  LAUNCH(2, 5) LAUNCH(3, 5) LAUNCH(4, 4)
 #undef LAUNCH
   return hipGetLastError();
 }

@@ -54,43 +54,6 @@ struct TriParams {
   int64_t total, per_xcd;
 };
 
-// Wave tiles of FM x FN fragments of 32 x 32 (2 x 2 waves per workgroup),
-// variant t of the "tile" option.
-constexpr int N_TRI_TILES = 8;
-__host__ __device__ inline int tri_fm(int t) {
-  const int v[N_TRI_TILES] = {2, 2, 3, 2, 3, 2, 3, 4};
-  return v[t];
-}
-__host__ __device__ inline int tri_fn(int t) {
-  const int v[N_TRI_TILES] = {2, 3, 2, 4, 3, 5, 5, 4};
-  return v[t];
-}
-// Relative rate of each variant (operand bytes per MFMA cycle fall with
-// FM FN / (FM + FN)).
-inline double tri_rate(int t) {
-  const double fm = tri_fm(t), fn = tri_fn(t);
-  return fm * fn / (fm + fn);
-}
-// Automatic variant: rate times the useful fraction of the padded
-// M = No^2 by N = nb No tile grid.
-inline int pick_tri_tile(int tile, int64_t No, int nb) {
-  if (tile >= 0 && tile < N_TRI_TILES) return tile;
-  double best = -1.0;
-  int pick = 0;
-  const double M = (double)No * No, N = (double)nb * No;
-  for (int t = 0; t < N_TRI_TILES; t++) {
-    const double mt = 64.0 * tri_fm(t), nt = 64.0 * tri_fn(t);
-    const double use = M / (__builtin_ceil(M / mt) * mt) *
-                       N / (__builtin_ceil(N / nt) * nt);
-    const double score = tri_rate(t) * use;
-    if (score > best) {
-      best = score;
-      pick = t;
-    }
-  }
-  return pick;
-}
-
 template <int FM, int FN>
 __global__ __launch_bounds__(256) void doubles_tri_kernel(TriParams P) {
   constexpr int MT = 64 * FM, NT = 64 * FN;
@@ -300,8 +263,8 @@ hipError_t launch_doubles_tri(const DevSystem &s, const BatchArgs &b,
   P.first = b.first;
   P.last = b.last;
   P.nb = b.nb;
-  tile = pick_tri_tile(tile, s.No, b.nb);
-  const int FM = tri_fm(tile), FN = tri_fn(tile);
+  tile = pick_t32_tile(tile, s.No, b.nb);
+  const int FM = t32_fm(tile), FN = t32_fn(tile);
   const int64_t MT = 64 * FM, NT = 64 * FN;
   P.mtiles = (int)((P.N2 + MT - 1) / MT);
   P.ntiles = (int)(((int64_t)b.nb * s.No + NT - 1) / NT);
