@@ -504,30 +504,32 @@ __global__ __launch_bounds__(256) void batched_kernel(BatchedLaunch P) {
       __syncthreads();  // class done before the next adds to W
     }
 
-    // energies, one tuple per wave; every wave runs the same trip count
+    // energies, one tuple per wave, each wave through its own slots: the
+    // singles table tsing[wave] is the wave's own, so its LDS writes need
+    // only to complete before the same wave reads them
 #pragma unroll 1
-    for (int base = 0; base < nb3; base += 4) {
-      const int slot = base + wave;
+    for (int slot = wave; slot < nb3; slot += 4) {
       const int64_t a = (int64_t)T.al * nb + slot / (nb * nb);
       const int64_t b = (int64_t)T.be * nb + (slot / nb) % nb;
       const int64_t c = (int64_t)T.ga * nb + slot % nb;
-      const bool ok = slot < nb3 && tuple_ok(S.Nv, a, b, c);
-      if (ok && lane < No) {
-        tsing[wave][0][lane] = S.Tai[a + S.Nv * lane];
-        tsing[wave][1][lane] = S.Tai[b + S.Nv * lane];
-        tsing[wave][2][lane] = S.Tai[c + S.Nv * lane];
+      const bool ok = tuple_ok(S.Nv, a, b, c);
+      double e = 0.0;
+      if (ok) {
+        if (lane < No) {
+          tsing[wave][0][lane] = S.Tai[a + S.Nv * lane];
+          tsing[wave][1][lane] = S.Tai[b + S.Nv * lane];
+          tsing[wave][2][lane] = S.Tai[c + S.Nv * lane];
+        }
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): LDS writes done
+        __builtin_amdgcn_wave_barrier();
+        e = tuple_energy(S, Gwin + (int64_t)slot * N2 * No, a, b, c,
+                         tsing[wave][0], tsing[wave][1], tsing[wave][2],
+                         lane);
+        __builtin_amdgcn_wave_barrier();  // reads done before the next writes
       }
-      __syncthreads();
-      if (slot < nb3) {
-        double e = 0.0;
-        if (ok)
-          e = tuple_energy(S, Gwin + (int64_t)slot * N2 * No, a, b, c,
-                           tsing[wave][0], tsing[wave][1], tsing[wave][2],
-                           lane);
-        if (lane == 0) etup[slot] = e;
-      }
-      __syncthreads();
+      if (lane == 0) etup[slot] = e;
     }
+    __syncthreads();  // every slot's energy is in etup
     if (P.tuple_e)
       for (int s = tid; s < nb3; s += 256) P.tuple_e[T.tuple_off + s] = etup[s];
     if (tid == 0) {

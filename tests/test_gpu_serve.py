@@ -127,7 +127,7 @@ def test_poisson_driver_reports_percentiles(hip_ctx):
 def test_mixed_precision_serving(hip_ctx):
     """A split-FP32 server: each request keeps only its split operands, and
     its energy is the same alone and in the mix and agrees with the plain
-    path's split-FP32 run."""
+    path's split-FP32 run to the FP32-accumulation level."""
     print("serve split_fp32", flush=True)
     mix = MIX[:4]
     alone = {}
@@ -149,4 +149,6 @@ def test_mixed_precision_serving(hip_ctx):
             ep, _ = hip_ctx.run(s, "split_fp32")
         finally:
             hip_ctx.release(s)
-        assert _close(alone[m], ep, REL_PLAIN)
+        # the plain path's split kernel accumulates on 32x32 MFMAs, the
+        # batched kernel behind the server on 16x16 ones
+        assert _close(alone[m], ep, 1e-7)
