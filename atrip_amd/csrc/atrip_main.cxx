@@ -1,13 +1,15 @@
 // `atrip` command-line driver: the counterpart of the reference's bench
-// driver (bench/main.cxx:403-502) for this build.  It fills CTF-shaped
-// tensors with the seeded synthetic generator (include/atrip_synth.h) and
-// calls atrip::Atrip::run, printing the reference's Energy lines.
+// driver (bench/main.cxx:403-502) for this build.  Each tensor is read from a
+// dense binary file when its path option names one, else filled by the seeded
+// synthetic generator (include/atrip_synth.h); then atrip::Atrip::run runs
+// and the reference's Energy lines are printed.
 #include <atrip.hpp>
 #include <atrip_synth.h>
 
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -20,7 +22,9 @@ void usage() {
          "             [--checkpoint-path P] [--nocheckpoint]\n"
          "             [--checkpoint-it K] [--checkpoint-% P] [--ijkabc]\n"
          "             [--cT] [--mixed] [--mixed-form split_fp32|fp32|bf16x3]\n"
-         "             [--complex] [--device D] [--chrono]\n";
+         "             [--complex] [--device D] [--chrono]\n"
+         "             [--ei P] [--ea P] [--Tph P] [--Tpphh P] [--Vpphh P]\n"
+         "             [--Vhhhp P] [--Vppph P] [--Jppph P] [--Jhhhp P]\n";
 }
 
 template <typename F>
@@ -53,6 +57,85 @@ CTF::Tensor<F> *make(int order, std::vector<int> lens, int tensor,
   return t;
 }
 
+bool file_exists(const std::string &path) {
+  return std::ifstream(path).good();
+}
+
+// out[p + h*q] = in[q + g*p] for p < h, q < g: the index halves of a tensor
+// swapped, as the driver's ijkabc assignments Tph["ia"] = iTph["ai"] and
+// Tpphh["ijab"] = iTpphh["abij"] (bench/main.cxx:300-306) do.
+template <typename F>
+void swap_halves(const CTF::Tensor<F> &in, CTF::Tensor<F> &out, int64_t h,
+                 int64_t g) {
+  std::vector<F> a((size_t)in.size()), b(a.size());
+  in.read_all(a.data());
+  for (int64_t q = 0; q < g; q++)
+    for (int64_t p = 0; p < h; p++) b[(size_t)(p + h * q)] = a[(size_t)(q + g * p)];
+  out.write_all(b.data());
+}
+
+// The reference driver's read_or_fill (bench/main.cxx:53-89): a tensor of
+// `lens` read from the dense column-major file `path` when it exists;
+// otherwise, with a warning if a path was given, the seeded synthetic fill.
+// With `flip` (ijkabc amplitudes) the file holds the tensor with its index
+// halves swapped (lens {no,nv} / {no,no,nv,nv}) and is transposed back.
+template <typename F>
+CTF::Tensor<F> *read_or_make(const char *name, int order, std::vector<int> lens,
+                             int tensor, uint64_t seed, int no, int nv,
+                             const std::string &path, bool flip = false) {
+  if (path.empty() || !file_exists(path)) {
+    if (!path.empty())
+      std::cout << "WARNING: file " << path << " provided but not found!\n"
+                << "Synthetic initialization for tensor " << name << "\n";
+    return make<F>(order, lens, tensor, seed, no, nv);
+  }
+  auto *t = new CTF::Tensor<F>(order, lens.data());
+  if (!flip) {
+    t->read_dense_from_file(path.c_str());
+    return t;
+  }
+  const int half = order / 2;
+  std::vector<int> fl(lens.begin() + half, lens.end());
+  fl.insert(fl.end(), lens.begin(), lens.begin() + half);
+  CTF::Tensor<F> f(order, fl.data());
+  f.read_dense_from_file(path.c_str());
+  int64_t h = 1, g = 1;
+  for (int d = 0; d < half; d++) h *= lens[d];
+  for (int d = half; d < order; d++) g *= lens[d];
+  swap_halves(f, *t, h, g);
+  return t;
+}
+
+// Orbital energies: the driver reads them as real tensors and converts them
+// to the field (bench/main.cxx:207-234).
+template <typename F>
+CTF::Tensor<F> *read_or_make_eps(const char *name, int n, int tensor,
+                                 uint64_t seed, int no, int nv,
+                                 const std::string &path) {
+  if (path.empty() || !file_exists(path))
+    return read_or_make<F>(name, 1, {n}, tensor, seed, no, nv, path);
+  CTF::Tensor<double> r(1, &n);
+  r.read_dense_from_file(path.c_str());
+  std::vector<double> v((size_t)n);
+  r.read_all(v.data());
+  std::vector<F> w(v.begin(), v.end());
+  auto *t = new CTF::Tensor<F>(1, &n);
+  t->write_all(w.data());
+  return t;
+}
+
+// Jabci / Jijka: the driver reads these from their files with no fallback
+// (bench/main.cxx:315-343); with no path given they are synthetic here.
+template <typename F>
+CTF::Tensor<F> *read_or_make_j(int order, std::vector<int> lens, int tensor,
+                               uint64_t seed, int no, int nv,
+                               const std::string &path) {
+  if (path.empty()) return make<F>(order, lens, tensor, seed, no, nv);
+  auto *t = new CTF::Tensor<F>(order, lens.data());
+  t->read_dense_from_file(path.c_str());
+  return t;
+}
+
 struct Options {
   int no = 0, nv = 0, device = 0;
   uint64_t seed = 1;
@@ -62,24 +145,33 @@ struct Options {
        chrono = false;
   std::string ck_path = "atrip-checkpoint.yaml";
   std::string mixed_form = "split_fp32";
+  std::string ei, ea, tph, tpphh, vpphh, vhhhp, vppph, jppph, jhhhp;
 };
 
 template <typename F>
-int run(const Options &o) {
+int run(const Options &o) try {
   using atrip::Atrip;
   const int no = o.no, nv = o.nv;
   const uint64_t seed = o.seed;
-  auto *ei = make<F>(1, {no}, ATRIP_SYN_EI, seed, no, nv);
-  auto *ea = make<F>(1, {nv}, ATRIP_SYN_EA, seed, no, nv);
-  auto *tai = make<F>(2, {nv, no}, ATRIP_SYN_TAI, seed, no, nv);
-  auto *tabij = make<F>(4, {nv, nv, no, no}, ATRIP_SYN_TABIJ, seed, no, nv);
-  auto *vabij = make<F>(4, {nv, nv, no, no}, ATRIP_SYN_VABIJ, seed, no, nv);
-  auto *vijka = make<F>(4, {no, no, no, nv}, ATRIP_SYN_VIJKA, seed, no, nv);
-  auto *vabci = make<F>(4, {nv, nv, nv, no}, ATRIP_SYN_VABCI, seed, no, nv);
+  const bool fl = o.ijkabc;
+  auto *ei = read_or_make_eps<F>("ei", no, ATRIP_SYN_EI, seed, no, nv, o.ei);
+  auto *ea = read_or_make_eps<F>("ea", nv, ATRIP_SYN_EA, seed, no, nv, o.ea);
+  auto *tai = read_or_make<F>("tph", 2, {nv, no}, ATRIP_SYN_TAI, seed, no, nv,
+                              o.tph, fl);
+  auto *tabij = read_or_make<F>("tpphh", 4, {nv, nv, no, no}, ATRIP_SYN_TABIJ,
+                                seed, no, nv, o.tpphh, fl);
+  auto *vabij = read_or_make<F>("Vpphh", 4, {nv, nv, no, no}, ATRIP_SYN_VABIJ,
+                                seed, no, nv, o.vpphh, fl);
+  auto *vijka = read_or_make<F>("Vhhhp", 4, {no, no, no, nv}, ATRIP_SYN_VIJKA,
+                                seed, no, nv, o.vhhhp);
+  auto *vabci = read_or_make<F>("Vppph", 4, {nv, nv, nv, no}, ATRIP_SYN_VABCI,
+                                seed, no, nv, o.vppph);
   CTF::Tensor<F> *jijka = nullptr, *jabci = nullptr;
-  if (o.ct) {
-    jijka = make<F>(4, {no, no, no, nv}, ATRIP_SYN_JIJKA, seed, no, nv);
-    jabci = make<F>(4, {nv, nv, nv, no}, ATRIP_SYN_JABCI, seed, no, nv);
+  if (o.ct || (!o.jppph.empty() && !o.jhhhp.empty())) {
+    jijka = read_or_make_j<F>(4, {no, no, no, nv}, ATRIP_SYN_JIJKA, seed, no,
+                              nv, o.jhhhp);
+    jabci = read_or_make_j<F>(4, {nv, nv, nv, no}, ATRIP_SYN_JABCI, seed, no,
+                              nv, o.jppph);
   }
   Atrip::init(0);
   auto in = typename Atrip::Input<F>()
@@ -104,18 +196,16 @@ int run(const Options &o) {
                 .with_mixed_precision(o.mixed)
                 .with_mixed_form(o.mixed_form)
                 .with_delete_Vppph(true);
-  try {
-    const auto out = Atrip::run(in);
-    std::printf("Energy: %.15g\nEnergy (cT): %.15g\n", out.energy,
-                out.ct_energy);
-  } catch (const std::string &e) {
-    std::cerr << "ERROR: " << e << "\n";
-    return 1;
-  } catch (const char *e) {
-    std::cerr << "ERROR: " << e << "\n";
-    return 1;
-  }
+  const auto out = Atrip::run(in);
+  std::printf("Energy: %.15g\nEnergy (cT): %.15g\n", out.energy,
+              out.ct_energy);
   return 0;
+} catch (const std::string &e) {
+  std::cerr << "ERROR: " << e << "\n";
+  return 1;
+} catch (const char *e) {
+  std::cerr << "ERROR: " << e << "\n";
+  return 1;
 }
 
 }  // namespace
@@ -150,6 +240,15 @@ int main(int argc, char **argv) {
     else if (a == "--complex") cplx = true;
     else if (a == "--device") o.device = std::atoi(next());
     else if (a == "--chrono") o.chrono = true;
+    else if (a == "--ei") o.ei = next();
+    else if (a == "--ea") o.ea = next();
+    else if (a == "--Tph") o.tph = next();
+    else if (a == "--Tpphh") o.tpphh = next();
+    else if (a == "--Vpphh") o.vpphh = next();
+    else if (a == "--Vhhhp") o.vhhhp = next();
+    else if (a == "--Vppph") o.vppph = next();
+    else if (a == "--Jppph") o.jppph = next();
+    else if (a == "--Jhhhp") o.jhhhp = next();
     else {
       usage();
       return 2;

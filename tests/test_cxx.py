@@ -3,6 +3,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -82,3 +83,105 @@ def test_cli_unknown_mixed_form(tmp_path):
                        cwd=tmp_path)
     assert r.returncode != 0
     assert "unknown mixed_form" in r.stdout + r.stderr
+
+
+# Tensor-file options of the driver, named as in the reference's
+# bench/main.cxx:473-494.
+FILE_OPT = {"ei": "--ei", "ea": "--ea", "Tai": "--Tph", "Tabij": "--Tpphh",
+            "Vabij": "--Vpphh", "Vijka": "--Vhhhp", "Vabci": "--Vppph",
+            "Jabci": "--Jppph", "Jijka": "--Jhhhp"}
+
+
+def _write_tensors(tmp_path, t, no, nv, flip=False):
+    """Dense column-major files of the tensors in `t` and the CLI options that
+    name them.  With `flip` the amplitudes are stored with their index halves
+    swapped (lens {no,nv} and {no,no,nv,nv}), the ijkabc input layout of the
+    reference driver (bench/main.cxx:263-306)."""
+    args = []
+    for name, arr in t.items():
+        a = arr
+        if flip and name in ("Tai", "Tabij", "Vabij"):
+            h, g = (nv, no) if name == "Tai" else (nv * nv, no * no)
+            a = a.reshape(g, h).T
+        path = tmp_path / f"{name}.bin"
+        np.ascontiguousarray(a).tofile(path)
+        args += [FILE_OPT[name], str(path)]
+    return args
+
+
+def _energies(r):
+    assert r.returncode == 0, r.stdout + r.stderr
+    e = float(re.search(r"^Energy: (\S+)", r.stdout, re.M).group(1))
+    ect = float(re.search(r"^Energy \(cT\): (\S+)", r.stdout, re.M).group(1))
+    return e, ect
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ijkabc", [False, True])
+def test_cli_reads_tensor_files(oracle, tmp_path, ijkabc):
+    """Tensors given as files replace the synthetic fill: files written from
+    seed 11, read by a run seeded 3, give the seed-11 energies.  Both J paths
+    given turn (cT) on, as in the reference driver (bench/main.cxx:315)."""
+    _need(CLI)
+    no, nv = 6, 11
+    t = oracle.synthetic(no, nv, 11, with_j=True)
+    args = _write_tensors(tmp_path, t, no, nv, flip=ijkabc)
+    r = subprocess.run([CLI, "--no", str(no), "--nv", str(nv), "--seed", "3",
+                        "--nocheckpoint"] + (["--ijkabc"] if ijkabc else []) + args,
+                       capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    e, ect = _energies(r)
+    e_ref, ect_ref = oracle.run(no, nv, t, ijkabc=ijkabc)
+    assert abs(e - e_ref) <= 1e-10 * abs(e_ref)
+    assert abs(ect - ect_ref) <= 1e-10 * abs(ect_ref)
+
+
+@pytest.mark.gpu
+def test_cli_reads_complex_tensor_files(oracle, tmp_path):
+    """Complex run from files: orbital energies as real doubles (the driver
+    reads them real and converts, bench/main.cxx:207-234), the other tensors
+    as interleaved complex doubles."""
+    _need(CLI)
+    no, nv = 5, 9
+    t = oracle.synthetic_z(no, nv, 11, with_j=True)
+    files = {k: (v.real if k in ("ei", "ea") else v) for k, v in t.items()}
+    args = _write_tensors(tmp_path, files, no, nv)
+    r = subprocess.run([CLI, "--no", str(no), "--nv", str(nv), "--seed", "3",
+                        "--complex", "--nocheckpoint"] + args,
+                       capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    e, ect = _energies(r)
+    e_ref, ect_ref = oracle.run_z(no, nv, t)
+    assert abs(e - e_ref) <= 1e-10 * abs(e_ref)
+    assert abs(ect - ect_ref) <= 1e-10 * abs(ect_ref)
+
+
+@pytest.mark.gpu
+def test_cli_missing_tensor_file_is_synthetic(oracle, tmp_path):
+    """A path that does not exist warns and falls back to the fill, as the
+    reference driver's read_or_fill does (bench/main.cxx:71-80)."""
+    _need(CLI)
+    no, nv, seed = 5, 10, 3
+    r = subprocess.run([CLI, "--no", str(no), "--nv", str(nv), "--seed", str(seed),
+                        "--nocheckpoint", "--Tph", str(tmp_path / "absent.bin")],
+                       capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    e, _ = _energies(r)
+    assert "WARNING: file" in r.stdout and "absent.bin" in r.stdout
+    e_ref, _ = oracle.run(no, nv, oracle.synthetic(no, nv, seed))
+    assert abs(e - e_ref) <= 1e-10 * abs(e_ref)
+
+
+def test_cli_tensor_file_errors(tmp_path):
+    """A file shorter than its tensor, and a J file that does not exist (the
+    driver reads J with no fallback), end the run with an error before any
+    device work."""
+    _need(CLI)
+    short = tmp_path / "short.bin"
+    np.zeros(3).tofile(short)
+    r = subprocess.run([CLI, "--no", "4", "--nv", "6", "--nocheckpoint",
+                        "--Tpphh", str(short)], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
+    assert r.returncode == 1 and "short read" in r.stderr
+    r = subprocess.run([CLI, "--no", "4", "--nv", "6", "--nocheckpoint",
+                        "--Jppph", str(tmp_path / "j1.bin"), "--Jhhhp",
+                        str(tmp_path / "j2.bin")], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
+    assert r.returncode == 1 and "cannot open" in r.stderr
