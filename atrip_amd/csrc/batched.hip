@@ -186,56 +186,35 @@ __device__ __forceinline__ void wave_tile(const BSys &S, int cl, int al,
                                                      acc[f][g], 0, 0, 0); \
   }
   // segments: 0 T3[x1] straight, 1 T3[x2] swapped, 2 H3[x2] straight,
-  //           3 H3[x1] swapped; K runs are multiples of 16.  One pipeline
-  // runs through all four: the load cursor (lsg, lk) moves to the next
-  // segment's pointers when it passes the end of one, so only the first
-  // chunk of the tile waits for memory.  Past the last chunk the cursor
-  // stays on it (a valid address), so every load is unconditional.
-  int lsg = 0;
-  int64_t lk = 0;
-#define SET_PTRS(SG)                                                          \
-  {                                                                           \
-    const int64_t K_ = (SG) < 2 ? Kv : Ko;                                    \
-    const double *Ab = (SG) < 2 ? S.T3 + ((SG) == 0 ? x1 : x2) * N2 * Kv      \
-                                : S.H3 + ((SG) == 2 ? x2 : x1) * N2 * Ko;     \
-    _Pragma("unroll") for (int f = 0; f < FM; f++) ap[f] =                    \
-        Ab + (int64_t)(((SG)&1) ? rowW[f] : rowS[f]) * K_ + 4 * kk;           \
-    _Pragma("unroll") for (int g = 0; g < FN; g++) bp[g] =                    \
-        bbase(S, cl, (SG), ca[g], cb[g], cc[g]) + (int64_t)crr[g] * K_ +      \
-        4 * kk;                                                               \
-  }
-#define ADVANCE()                                                             \
-  {                                                                           \
-    lk += KCHUNK;                                                             \
-    if (lk >= (lsg < 2 ? Kv : Ko)) {                                          \
-      if (lsg == 3) {                                                         \
-        lk -= KCHUNK;                                                         \
-      } else {                                                                \
-        lsg++;                                                                \
-        lk = 0;                                                               \
-        SET_PTRS(lsg);                                                        \
-      }                                                                       \
-    }                                                                         \
-  }
-  SET_PTRS(0);
-  const int nchunks = (int)(2 * (Kv / KCHUNK) + 2 * (Ko / KCHUNK));
-  LOAD(a0, b0, lk);
-  ADVANCE();
+  //           3 H3[x1] swapped; K runs are multiples of 16.  Each segment
+  // has its own two-stage pipeline: one pipeline through all four keeps the
+  // next segment's pointers live as well and costs the kernel its second
+  // wave per SIMD.
 #pragma unroll 1
-  for (int c = 0;; c += 2) {
-    LOAD(a1, b1, lk);
-    ADVANCE();
-    COMPUTE(a0, b0);
-    if (c + 1 >= nchunks) break;
-    LOAD(a0, b0, lk);
-    ADVANCE();
-    COMPUTE(a1, b1);
-    if (c + 2 >= nchunks) break;
+  for (int sg = 0; sg < 4; sg++) {
+    const int64_t K_ = sg < 2 ? Kv : Ko;
+    const double *Ab = sg < 2 ? S.T3 + (sg == 0 ? x1 : x2) * N2 * Kv
+                              : S.H3 + (sg == 2 ? x2 : x1) * N2 * Ko;
+#pragma unroll
+    for (int f = 0; f < FM; f++)
+      ap[f] = Ab + (int64_t)((sg & 1) ? rowW[f] : rowS[f]) * K_ + 4 * kk;
+#pragma unroll
+    for (int g = 0; g < FN; g++)
+      bp[g] = bbase(S, cl, sg, ca[g], cb[g], cc[g]) + (int64_t)crr[g] * K_ +
+              4 * kk;
+    const int64_t last = K_ - KCHUNK;
+    LOAD(a0, b0, 0);
+    for (int64_t k0 = 0;; k0 += 2 * KCHUNK) {
+      LOAD(a1, b1, k0 + KCHUNK < last ? k0 + KCHUNK : last);
+      COMPUTE(a0, b0);
+      if (k0 + KCHUNK > last) break;
+      LOAD(a0, b0, k0 + 2 * KCHUNK < last ? k0 + 2 * KCHUNK : last);
+      COMPUTE(a1, b1);
+      if (k0 + 2 * KCHUNK > last) break;
+    }
   }
 #undef LOAD
 #undef COMPUTE
-#undef SET_PTRS
-#undef ADVANCE
   store_tile<FM, FN, 8>(acc, cl, mt, nt, No, cvalid, cslot, crr, Gwin, add,
                         lane, xp);
 }
@@ -443,7 +422,7 @@ __device__ __forceinline__ T uniform_load(const T *p) {
 }
 
 template <int PREC>
-__global__ __launch_bounds__(256) void batched_kernel(BatchedLaunch P) {
+__global__ __launch_bounds__(256, 2) void batched_kernel(BatchedLaunch P) {
   __shared__ int task_sh;
   __shared__ double etup[512];
   __shared__ double tsing[4][3][BMAX_NO];
