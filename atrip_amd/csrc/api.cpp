@@ -1010,7 +1010,7 @@ int atrip_hip_set_option(atrip_hip_ctx *ctx, const char *key, int64_t value) {
       ctx->batch_nb = (int)value;
     } else if (k == "batch_tile") {
       if (value < -1 || value >= N_BTILES)
-        throw Fail{"batch_tile must be in -1..7"};
+        throw Fail{"batch_tile must be in -1.." + std::to_string(N_BTILES - 1)};
       ctx->batch_tile = (int)value;
     } else {
       throw Fail{"unknown option '" + k + "'"};

@@ -475,6 +475,8 @@ __global__ __launch_bounds__(256, 2) void batched_kernel(BatchedLaunch P) {
           TILE_CASE(5)
           TILE_CASE(6)
           TILE_CASE(7)
+          TILE_CASE(8)
+          TILE_CASE(9)
 #undef TILE_CASE
           default:
             break;

@@ -325,13 +325,13 @@ struct BSys {
   int64_t No, Nv, Kv, Ko, npoints;
   int32_t nb, tile;
 };
-constexpr int N_BTILES = 8;
+constexpr int N_BTILES = 10;
 constexpr __host__ __device__ inline int bfm(int v) {
-  constexpr int t[N_BTILES] = {1, 1, 2, 2, 1, 4, 2, 4};
+  constexpr int t[N_BTILES] = {1, 1, 2, 2, 1, 4, 2, 4, 3, 3};
   return t[v];
 }
 constexpr __host__ __device__ inline int bfn(int v) {
-  constexpr int t[N_BTILES] = {1, 2, 1, 2, 4, 1, 4, 2};
+  constexpr int t[N_BTILES] = {1, 2, 1, 2, 4, 1, 4, 2, 2, 3};
   return t[v];
 }
 

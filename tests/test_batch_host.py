@@ -49,7 +49,7 @@ def test_batch_model_choices():
 
     for no, nv, _ in workloads.batch_shapes(64):
         nb, fm, fn = hip.batch_model(no, nv)
-        assert nb in (1, 2, 4, 8) and fm in (1, 2, 4) and fn in (1, 2, 4)
+        assert nb in (1, 2, 4, 8) and fm in (1, 2, 3, 4) and fn in (1, 2, 3, 4)
         # the chosen window never needs more than the scratch bound
         assert nb == 1 or nb ** 3 * no ** 3 * 8 * 256 <= 8192 * 2 ** 20
     # a tight scratch bound forces small windows

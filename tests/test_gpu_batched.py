@@ -16,6 +16,7 @@ pytestmark = pytest.mark.gpu
 
 REL_ORACLE = 1e-10
 REL_PLAIN = 1e-12
+N_TILES = 10  # wave-tile variants of the batched kernel (batch_tile 0..9)
 
 SHAPES = [(4, 20, 1000), (5, 23, 1001), (7, 31, 1002), (16, 40, 1003),
           (9, 57, 1004), (12, 33, 1005), (1, 5, 1006), (2, 3, 1007),
@@ -89,7 +90,7 @@ def test_batched_tiles_and_windows(hip_ctx, batch):
         for nb in (1, 2, 4, 8):
             hip_ctx.set_option("batch_nb", nb)
             first = None
-            for tile in range(8):
+            for tile in range(N_TILES):
                 print(f"  nb={nb} tile={tile}", flush=True)
                 hip_ctx.set_option("batch_tile", tile)
                 e = hip_ctx.batch_run(batch)
@@ -110,7 +111,7 @@ def test_batched_tuner(hip_ctx, batch):
     for s in batch:
         p = hip_ctx.batch_plan(s)
         assert p["measured"], p
-        assert p["nb"] in (1, 2, 4, 8) and p["fm"] in (1, 2, 4) and p["fn"] in (1, 2, 4)
+        assert p["nb"] in (1, 2, 4, 8) and p["fm"] in (1, 2, 3, 4) and p["fn"] in (1, 2, 3, 4)
         assert p["sec_window"] > 0
     e = hip_ctx.batch_run(batch)
     for q in range(len(batch)):
