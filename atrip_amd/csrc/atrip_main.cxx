@@ -21,7 +21,7 @@ void usage() {
       << "usage: atrip --no N --nv N [--seed S] [--max-iterations K]\n"
          "             [--checkpoint-path P] [--nocheckpoint]\n"
          "             [--checkpoint-it K] [--checkpoint-% P] [--ijkabc]\n"
-         "             [--cT] [--mixed] [--mixed-form split_fp32|fp32|bf16x3]\n"
+         "             [--cT] [--mixed] [--mixed-form bf16x3|split_fp32|fp32]\n"
          "             [--complex] [--device D] [--chrono]\n"
          "             [--ei P] [--ea P] [--Tph P] [--Tpphh P] [--Vpphh P]\n"
          "             [--Vhhhp P] [--Vppph P] [--Jppph P] [--Jhhhp P]\n";
@@ -144,7 +144,7 @@ struct Options {
   bool nockpt = false, ijkabc = false, ct = false, mixed = false,
        chrono = false;
   std::string ck_path = "atrip-checkpoint.yaml";
-  std::string mixed_form = "split_fp32";
+  std::string mixed_form = "bf16x3";
   std::string ei, ea, tph, tpphh, vpphh, vhhhp, vppph, jppph, jhhhp;
 };
 

@@ -18,7 +18,7 @@ FP64 = 0
 SPLIT_FP32 = 1
 FP32 = 2
 BF16X3 = 3
-PRECISIONS = {"fp64": FP64, "split_fp32": SPLIT_FP32, "mixed": SPLIT_FP32,
+PRECISIONS = {"fp64": FP64, "split_fp32": SPLIT_FP32,
               "fp32": FP32, "bf16x3": BF16X3}
 
 # Exported C-ABI symbols (kept in sync with include/atrip_hip.h).

@@ -123,9 +123,11 @@ struct Atrip {
     // Additions of this build (no counterpart in the reference):
     ATRIP_ADD_ATTRIBUTE(int, device, 0)          // HIP device index
     ATRIP_ADD_ATTRIBUTE(bool, mixed_precision, false)  // mixed doubles:
-    // "split_fp32" (hi*hi + hi*lo + lo*hi on FP32 MFMA), "fp32" (hi*hi only)
-    // or "bf16x3" (six bf16 part products); FP64 energy in every form
-    ATRIP_ADD_ATTRIBUTE(std::string, mixed_form, "split_fp32")
+    // "bf16x3" (three bf16 parts per operand, six part products on bf16
+    // MFMA: the fastest and most accurate), "split_fp32" (hi*hi on FP32
+    // MFMA plus hi*lo + lo*hi on bf16 MFMA) or "fp32" (hi*hi only); FP32
+    // accumulation and FP64 energy in every form
+    ATRIP_ADD_ATTRIBUTE(std::string, mixed_form, "bf16x3")
   };
 
   struct Output {

@@ -162,7 +162,10 @@ int main(int argc, char **argv) {
            assert(in.read_checkpoint_if_exists);
            assert(in.checkpoint_at_percentage == 10);
            assert(!in.ijkabc);
-           assert(in.with_max_iterations(7).max_iterations == 7);)
+           assert(in.with_max_iterations(7).max_iterations == 7);
+           // additions of this build: FP64 unless asked, bf16x3 when mixed
+           assert(!in.mixed_precision);
+           assert(in.mixed_form == "bf16x3");)
 
   if (!gpu) {
     std::cout << "host cases passed\n";
