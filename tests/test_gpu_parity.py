@@ -152,6 +152,47 @@ def test_mid_size_sample(hip_ctx, oracle):
         hip_ctx.release(s)
 
 
+def test_benchmark_size_sample_and_properties(hip_ctx, oracle):
+    """No=40, Nv=400, seed 1: the size and inputs of the FP64 benchmark
+    (Vabci 20.5 GB).  Sampled tuple ranges at the start, middle and end of
+    the tuple space against the oracle; over a larger range, the
+    size-independent properties: the per-tuple energies do not depend on the
+    window edge, batch or tile (bitwise), and every mixed form stays within
+    the mixed tolerance of FP64 tuple by tuple."""
+    no, nv, seed = 40, 400, 1
+    n = oracle.n_tuples(nv)
+    ranges = [(0, 96), (n // 2, 96), (n - 96, 96)]
+    first, count = 5_000_000, 20_000
+    print("benchmark size: upload", flush=True)
+    s = hip_ctx.upload_synthetic(no, nv, seed)
+    try:
+        sampled = [hip_ctx.tuple_energies(s, f, c) for f, c in ranges]
+        base = hip_ctx.tuple_energies(s, first, count)
+        for window, batch, tile in [(4, 1, 0), (8, 2, 3)]:
+            print("benchmark size: window/batch/tile", window, batch, tile, flush=True)
+            hip_ctx.set_option("window", window)
+            hip_ctx.set_option("batch", batch)
+            hip_ctx.set_option("tile", tile)
+            assert np.array_equal(hip_ctx.tuple_energies(s, first, count), base)
+        hip_ctx.set_option("tile", -1)
+        hip_ctx.set_option("window", 8)
+        hip_ctx.set_option("batch", 4)
+        for precision in MIXED_FORMS:
+            print("benchmark size:", precision, flush=True)
+            pm = hip_ctx.tuple_energies(s, first, count, precision=precision)
+            assert np.max(np.abs(pm - base)) <= MIXED_REL_TUPLE * np.max(np.abs(base))
+    finally:
+        hip_ctx.release(s)
+    print("benchmark size: oracle sample", flush=True)
+    t = oracle.synthetic(no, nv, seed, nthreads=16)
+    oracle.blas_init()
+    for (f, c), pt in zip(ranges, sampled):
+        _, _, pt_ref, _ = oracle.run(no, nv, t, first=f, count=c, use_gemm=True,
+                                     nthreads=16, per_tuple=True)
+        assert np.max(np.abs(pt - pt_ref)) <= REL_TUPLE * np.max(np.abs(pt_ref)), (f, c)
+    del t
+
+
 def test_errors_are_reported(hip_ctx):
     from atrip_amd import hip
 
