@@ -272,9 +272,9 @@ def main(argv=None):
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(tuples would shard by index range with one scalar all-reduce; "
-              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (tuples "
+              "would shard by FLOP-balanced index ranges, one per GPU, joined by "
+              "one scalar all-reduce of E(T)). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     from atrip_amd import hip as H
