@@ -517,11 +517,29 @@ struct RunResult {
   double e = 0, ect = 0;
 };
 
+// Destroys the queued timing events (read at the end of a run, or left by
+// a run that failed); each event can end one pair and start the next, so
+// they are deduplicated first.
+void drop_pending_events(atrip_hip_ctx *ctx) {
+  std::vector<hipEvent_t> uniq;
+  for (auto &pr : ctx->pending_events) {
+    uniq.push_back(pr.first);
+    uniq.push_back(pr.second);
+  }
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  for (auto ev : uniq) (void)hipEventDestroy(ev);
+  ctx->pending_events.clear();
+  ctx->pending_names.clear();
+  ctx->pending_flop.clear();
+}
+
 // Runs every batch of windows intersecting [first, last).  When `sink` is
 // given it receives the per-slot energies of each batch.
 template <typename Sink>
 RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
                     uint64_t first, uint64_t last, Sink &&sink) {
+  drop_pending_events(ctx);
   DevSystem &s = get_sys(ctx, sys);
   WindowPlan &plan = get_plan(ctx, sys, s);
   const uint64_t n = n_tuples(s.Nv);
@@ -627,7 +645,18 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
     b.slots = (int64_t)b.nw * nb3;
     uint64_t cnt = 0;
     for (int w = 0; w < b.nw; w++) cnt += counts[w0 + w];
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    // the batch's timing events; destroyed here unless handed to the
+    // context's pending list (a launch that throws leaves none behind)
+    struct BatchEvents {
+      hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+      bool handed = false;
+      ~BatchEvents() {
+        if (!handed)
+          for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+      }
+    } ev;
+    hipEvent_t &e0 = ev.e[0], &e1 = ev.e[1], &e2 = ev.e[2];
     if (ctx->timing) {
       check(hipEventCreate(&e0), "event");
       check(hipEventCreate(&e1), "event");
@@ -674,6 +703,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
       ctx->pending_events.push_back({e1, e2});
       ctx->pending_names.push_back("energy");
       ctx->pending_flop.push_back(0.0);
+      ev.handed = true;
     }
     check(launch_batch_reduce(s, b, ctx->e_slot, ctx->partials, bi,
                               ctx->stream),
@@ -719,18 +749,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
     k.ms += ms;
     k.flop += ctx->pending_flop[q];
   }
-  // each event appears as the end of one pair and the start of another
-  std::vector<hipEvent_t> uniq;
-  for (auto &pr : ctx->pending_events) {
-    uniq.push_back(pr.first);
-    uniq.push_back(pr.second);
-  }
-  std::sort(uniq.begin(), uniq.end());
-  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
-  for (auto ev : uniq) (void)hipEventDestroy(ev);
-  ctx->pending_events.clear();
-  ctx->pending_names.clear();
-  ctx->pending_flop.clear();
+  drop_pending_events(ctx);  // read: release them
   return res;
 }
 
