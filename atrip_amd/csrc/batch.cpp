@@ -255,7 +255,14 @@ LaunchOut run_tasks(atrip_hip_ctx *ctx, int precision,
   L.task_e = dtask_e.make<double>(2 * tasks.size(), "alloc task energies");
   L.tuple_e = nullptr;
   L.grid = grid;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  struct Events {  // destroyed on every exit, a failed launch included
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  hipEvent_t &e0 = ev.e[0], &e1 = ev.e[1];
   if (timed) {
     check(hipEventCreate(&e0), "event");
     check(hipEventCreate(&e1), "event");
@@ -269,11 +276,7 @@ LaunchOut run_tasks(atrip_hip_ctx *ctx, int precision,
                        hipMemcpyDeviceToHost, st),
         "download window energies");
   check(hipStreamSynchronize(st), "batched run");
-  if (timed) {
-    check(hipEventElapsedTime(&out.ms, e0, e1), "elapsed");
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-  }
+  if (timed) check(hipEventElapsedTime(&out.ms, e0, e1), "elapsed");
   out.grid = grid;
   return out;
 }
