@@ -19,6 +19,7 @@ namespace atrip_amd {
 // Windows (block triples of edge nb) holding at least one tuple, in
 // lexicographic order, with their tuple counts.
 std::vector<Win> windows_of(int64_t Nv, int nb) {
+  if (nb < 1) throw Fail{"window edge must be positive"};
   std::vector<Win> out;
   const int nblk = (int)((Nv + nb - 1) / nb);
   for (int al = 0; al < nblk; al++)
@@ -90,6 +91,8 @@ double model_sec_tuple(int64_t No, int64_t Nv, int nb, int tile,
 BatchTune model_pick(int64_t No, int64_t Nv, int64_t scratch_mb, int grid,
                      int precision) {
   BatchTune best;
+  // no tuple (Nv = 1): nothing to rank; any valid window edge yields no work
+  if (n_tuples((uint64_t)Nv) == 0) return best;
   double best_s = 1e300;
   for (int nb : NB_CAND) {
     if (!nb_fits(No, nb, scratch_mb, grid)) continue;

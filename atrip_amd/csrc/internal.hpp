@@ -180,7 +180,7 @@ struct Batch;
 
 // Tuned plan of the batched form for one system shape.
 struct BatchTune {
-  int nb = 0, tile = 0;
+  int nb = 1, tile = 0;
   double sec_window = 0.0;  // measured (or modelled) time of a full window
   bool measured = false;
 };

@@ -44,6 +44,14 @@ def test_flop_and_tuple_counts():
     assert workloads.doubles_flop(10, 50) == pytest.approx(1.588e10, rel=1e-3)
 
 
+def test_batch_model_empty_tuple_space():
+    """Nv = 1 holds no tuple: the model still returns a valid window edge."""
+    from atrip_amd import hip
+
+    nb, fm, fn = hip.batch_model(3, 1)
+    assert nb >= 1 and fm >= 1 and fn >= 1
+
+
 def test_batch_model_choices():
     from atrip_amd import hip
 

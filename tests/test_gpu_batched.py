@@ -134,6 +134,24 @@ def test_batched_seeded_sample(hip_ctx):
             hip_ctx.release(s)
 
 
+def test_batched_edge_sizes(hip_ctx):
+    """The largest No the batched form takes (64) and an empty tuple space
+    (Nv = 1: no a <= b <= c other than a = b = c) in one launch with an
+    ordinary system: each equals the plain path, the empty one exactly 0."""
+    shapes = [(64, 4, 1010), (3, 1, 1011), (6, 9, 1012)]
+    print("batched edge sizes", shapes, flush=True)
+    systems = [hip_ctx.upload_synthetic(no, nv, seed) for no, nv, seed in shapes]
+    try:
+        e = hip_ctx.batch_run(systems)
+        for s, ei, sh in zip(systems, e, shapes):
+            ep, _ = hip_ctx.run(s)
+            assert _close(ei, ep, REL_PLAIN), (sh, ei, ep)
+        assert e[1] == 0.0 and hip_ctx.run(systems[1])[0] == 0.0
+    finally:
+        for s in systems:
+            hip_ctx.release(s)
+
+
 def test_batched_errors(hip_ctx):
     assert len(hip_ctx.batch_run([])) == 0
     with pytest.raises(hip.HipError, match="unknown system"):

@@ -66,6 +66,27 @@ def test_same_alone_and_in_the_mix(hip_ctx, alone, policy, iter_ms):
         assert r["t_submit"] <= r["t_admit"] <= r["t_done"]
 
 
+def test_empty_request_completes(hip_ctx):
+    """A request with no tuple (Nv = 1) retires at admission with E(T) = 0,
+    next to an ordinary request whose result it does not change."""
+    with serve.Server(hip_ctx, hbm_budget_gb=50.0, iter_ms=10.0) as srv:
+        empty = srv.submit_synthetic(3, 1, 5100)
+        other = srv.submit_synthetic(5, 9, 5101)
+        for _ in range(1000):
+            srv.step()
+            if all(srv.result(r)["state"] == serve.DONE for r in (empty, other)):
+                break
+        assert srv.result(empty)["state"] == serve.DONE
+        assert srv.result(empty)["energy"] == 0.0
+        assert srv.result(other)["state"] == serve.DONE
+        e_other = srv.result(other)["energy"]
+    s = hip_ctx.upload_synthetic(5, 9, 5101)
+    try:
+        assert _close(e_other, hip_ctx.run(s)[0], 1e-12)
+    finally:
+        hip_ctx.release(s)
+
+
 def test_budget_admission(hip_ctx, alone):
     """A budget that holds about two of the systems: requests wait for
     memory, the resident bytes stay under the budget, results unchanged."""
