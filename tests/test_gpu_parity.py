@@ -193,6 +193,18 @@ def test_benchmark_size_sample_and_properties(hip_ctx, oracle):
     del t
 
 
+def test_empty_tuple_space(hip_ctx):
+    """Nv = 1 holds no a <= b <= c other than a = b = c: E(T) = E(cT) = 0
+    in every form, and an empty tuple range returns no energies."""
+    s = hip_ctx.upload_synthetic(3, 1, 9, with_j=True)
+    try:
+        for precision in ["fp64"] + MIXED_FORMS:
+            assert hip_ctx.run(s, precision) == (0.0, 0.0), precision
+        assert hip_ctx.tuple_energies(s, 0, 0).size == 0
+    finally:
+        hip_ctx.release(s)
+
+
 def test_errors_are_reported(hip_ctx):
     from atrip_amd import hip
 
