@@ -62,6 +62,12 @@ __device__ __forceinline__ const double *bbase(const BSys &S, int cl, int sg, in
 // i per row, 128 bytes).  For classes w and v the fragment is first
 // transposed through the wave's 16 x 17 LDS area so that a store
 // instruction covers 16 consecutive rows m (consecutive i) of 4 columns.
+// Per column group the element offsets come first and every old value
+// (classes v and u add) is loaded before any store, so the adds wait on
+// memory once per group instead of once per element (a load after a store
+// to a possibly aliasing address cannot be moved above it).  Every element
+// of W receives one contribution per class, so within a class the tiles'
+// elements are disjoint.
 // OUT = 8: FP64 MFMA layout D[i][j] in lane j + 16 (i % 4), register i / 4;
 // OUT = 4: FP32 MFMA layout D[i][j] in lane j + 16 (i / 4), register i % 4.
 template <int FM, int FN, int OUT, typename V4>
@@ -75,46 +81,65 @@ __device__ __forceinline__ void store_tile(const V4 (&acc)[FM][FN], int cl,
   const int no = (int)No, n2 = no * no, w3 = n2 * no;
 #pragma unroll
   for (int g = 0; g < FN; g++) {
+    int off[FM][4];  // element offset in the window's W, -1 if none
+    if (cl == 2) {
 #pragma unroll
-    for (int f = 0; f < FM; f++) {
-      if (cl == 2) {
-        if (!cvalid[g]) continue;
-        double *Gs = Gwin + cslot[g] * w3;
+      for (int f = 0; f < FM; f++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int row = OUT == 8 ? 4 * r + kk : 4 * kk + r;
           const int m = mt * 16 * FM + f * 16 + row;
-          if (m >= n2) continue;
           const int p = m % no, q = m / no;
-          double *dst = Gs + crr[g] + no * (p + no * q);
-          const double v = (double)acc[f][g][r];
-          *dst = add ? *dst + v : v;
+          off[f][r] = cvalid[g] && m < n2
+                          ? cslot[g] * w3 + crr[g] + no * (p + no * q)
+                          : -1;
         }
-        continue;
-      }
+    } else {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int row = OUT == 8 ? 4 * r + kk : 4 * kk + r;
-        xp[row * 17 + l16] = (double)acc[f][g][r];
-      }
-      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): LDS writes done
-      __builtin_amdgcn_wave_barrier();
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int col = 4 * r + kk, row = l16;
-        const double v = xp[row * 17 + col];
+        const int col = 4 * r + kk;
         const int slot = __shfl(cslot[g], col, 64);
         const int rr = __shfl(crr[g], col, 64);
         const bool ok = __shfl((int)cvalid[g], col, 64) != 0;
-        const int m = mt * 16 * FM + f * 16 + row;
-        if (!ok || m >= n2) continue;
-        const int p = m % no, q = m / no;
-        const int off = cl == 0 ? p + no * (q + no * rr) : p + no * (rr + no * q);
-        double *dst = Gwin + slot * w3 + off;
-        *dst = add ? *dst + v : v;
+#pragma unroll
+        for (int f = 0; f < FM; f++) {
+          const int m = mt * 16 * FM + f * 16 + l16;
+          const int p = m % no, q = m / no;
+          const int o = cl == 0 ? p + no * (q + no * rr) : p + no * (rr + no * q);
+          off[f][r] = ok && m < n2 ? slot * w3 + o : -1;
+        }
       }
-      __builtin_amdgcn_s_waitcnt(0xC07F);  // reads done before the next tile
-      __builtin_amdgcn_wave_barrier();
+    }
+    double old[FM][4];
+    if (add) {
+#pragma unroll
+      for (int f = 0; f < FM; f++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)  // always a valid address
+          old[f][r] = Gwin[off[f][r] >= 0 ? off[f][r] : 0];
+    }
+#pragma unroll
+    for (int f = 0; f < FM; f++) {
+      double v[4];
+      if (cl == 2) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[r] = (double)acc[f][g][r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int row = OUT == 8 ? 4 * r + kk : 4 * kk + r;
+          xp[row * 17 + l16] = (double)acc[f][g][r];
+        }
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): LDS writes done
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[r] = xp[l16 * 17 + 4 * r + kk];
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // reads done before the next tile
+        __builtin_amdgcn_wave_barrier();
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (off[f][r] >= 0) Gwin[off[f][r]] = add ? old[f][r] + v[r] : v[r];
     }
   }
 }
