@@ -63,9 +63,10 @@ __device__ __forceinline__ const double *bbase(const BSys &S, int cl, int sg, in
 // transposed through the wave's 16 x 17 LDS area so that a store
 // instruction covers 16 consecutive rows m (consecutive i) of 4 columns.
 // Per column group the element offsets come first and every old value
-// (classes v and u add) is loaded before any store, so the adds wait on
-// memory once per group instead of once per element (a load after a store
-// to a possibly aliasing address cannot be moved above it).  Every element
+// (classes v and u add) is loaded before any store, the next group's
+// before this group's stores, so the adds wait on memory about once per
+// tile instead of once per element (a load after a store to a possibly
+// aliasing address cannot be moved above it).  Every element
 // of W receives one contribution per class, so within a class the tiles'
 // elements are disjoint.
 // OUT = 8: FP64 MFMA layout D[i][j] in lane j + 16 (i % 4), register i / 4;
@@ -79,45 +80,47 @@ __device__ __forceinline__ void store_tile(const V4 (&acc)[FM][FN], int cl,
                                            bool add, int lane, double *xp) {
   const int kk = lane >> 4, l16 = lane & 15;
   const int no = (int)No, n2 = no * no, w3 = n2 * no;
+  int off[2][FM][4];     // element offsets in the window's W, -1 if none
+  double old[2][FM][4];  // their old values (classes v and u)
+  // offsets and old values of column group g into set h
+#define GROUP_LOAD(g, h)                                                       \
+  {                                                                            \
+    if (cl == 2) {                                                             \
+      _Pragma("unroll") for (int f = 0; f < FM; f++)                           \
+          _Pragma("unroll") for (int r = 0; r < 4; r++) {                      \
+        const int row = OUT == 8 ? 4 * r + kk : 4 * kk + r;                    \
+        const int m = mt * 16 * FM + f * 16 + row;                             \
+        const int p = m % no, q = m / no;                                      \
+        off[h][f][r] = cvalid[g] && m < n2                                     \
+                           ? cslot[g] * w3 + crr[g] + no * (p + no * q)        \
+                           : -1;                                               \
+      }                                                                        \
+    } else {                                                                   \
+      _Pragma("unroll") for (int r = 0; r < 4; r++) {                          \
+        const int col = 4 * r + kk;                                            \
+        const int slot = __shfl(cslot[g], col, 64);                            \
+        const int rr = __shfl(crr[g], col, 64);                                \
+        const bool ok = __shfl((int)cvalid[g], col, 64) != 0;                  \
+        _Pragma("unroll") for (int f = 0; f < FM; f++) {                       \
+          const int m = mt * 16 * FM + f * 16 + l16;                           \
+          const int p = m % no, q = m / no;                                    \
+          const int o =                                                        \
+              cl == 0 ? p + no * (q + no * rr) : p + no * (rr + no * q);       \
+          off[h][f][r] = ok && m < n2 ? slot * w3 + o : -1;                    \
+        }                                                                      \
+      }                                                                        \
+    }                                                                          \
+    if (add)                                                                   \
+      _Pragma("unroll") for (int f = 0; f < FM; f++)                           \
+          _Pragma("unroll") for (int r = 0; r < 4; r++) old[h][f][r] =         \
+              Gwin[off[h][f][r] >= 0 ? off[h][f][r] : 0];                      \
+  }
+  GROUP_LOAD(0, 0)
 #pragma unroll
   for (int g = 0; g < FN; g++) {
-    int off[FM][4];  // element offset in the window's W, -1 if none
-    if (cl == 2) {
-#pragma unroll
-      for (int f = 0; f < FM; f++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int row = OUT == 8 ? 4 * r + kk : 4 * kk + r;
-          const int m = mt * 16 * FM + f * 16 + row;
-          const int p = m % no, q = m / no;
-          off[f][r] = cvalid[g] && m < n2
-                          ? cslot[g] * w3 + crr[g] + no * (p + no * q)
-                          : -1;
-        }
-    } else {
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int col = 4 * r + kk;
-        const int slot = __shfl(cslot[g], col, 64);
-        const int rr = __shfl(crr[g], col, 64);
-        const bool ok = __shfl((int)cvalid[g], col, 64) != 0;
-#pragma unroll
-        for (int f = 0; f < FM; f++) {
-          const int m = mt * 16 * FM + f * 16 + l16;
-          const int p = m % no, q = m / no;
-          const int o = cl == 0 ? p + no * (q + no * rr) : p + no * (rr + no * q);
-          off[f][r] = ok && m < n2 ? slot * w3 + o : -1;
-        }
-      }
-    }
-    double old[FM][4];
-    if (add) {
-#pragma unroll
-      for (int f = 0; f < FM; f++)
-#pragma unroll
-        for (int r = 0; r < 4; r++)  // always a valid address
-          old[f][r] = Gwin[off[f][r] >= 0 ? off[f][r] : 0];
-    }
+    const int h = g & 1;
+    // the next group's old values are requested before this group's stores
+    if (g + 1 < FN) GROUP_LOAD(g + 1, h ^ 1)
 #pragma unroll
     for (int f = 0; f < FM; f++) {
       double v[4];
@@ -139,9 +142,11 @@ __device__ __forceinline__ void store_tile(const V4 (&acc)[FM][FN], int cl,
       }
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        if (off[f][r] >= 0) Gwin[off[f][r]] = add ? old[f][r] + v[r] : v[r];
+        if (off[h][f][r] >= 0)
+          Gwin[off[h][f][r]] = add ? old[h][f][r] + v[r] : v[r];
     }
   }
+#undef GROUP_LOAD
 }
 
 // One wave tile: rows mt*16*FM .. of the No^2 (p,q) rows, columns
