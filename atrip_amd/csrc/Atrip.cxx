@@ -49,36 +49,53 @@ struct Ctx {
 };
 
 template <typename F>
-std::vector<F> read_tensor(CTF::Tensor<F> *t, const char *name,
-                           size_t expect) {
+void check_size(CTF::Tensor<F> *t, const char *name, size_t expect) {
   if (!t) throw std::string("atrip: missing input tensor ") + name;
   size_t n = 1;
   for (int d = 0; d < t->order; d++) n *= (size_t)t->lens[d];
   if (n != expect)
     throw std::string("atrip: tensor ") + name + " has " + std::to_string(n) +
         " elements, expected " + std::to_string(expect);
-  std::vector<F> v(n);
-  t->read_all(v.data());
-  return v;
 }
 
-// Host memory of a tensor as the C-ABI sees it: doubles, interleaved
-// (re, im) for complex tensors (std::complex<double> is two doubles).
+// Elements [first, first + count) of a tensor in column-major order.  The
+// shim serves the contiguous range directly; with Cyclops the range is
+// asked for by global index, CTF's Tensor::read(npair, global_idx, data).
 template <typename F>
-const double *raw(const std::vector<F> &v) {
-  return reinterpret_cast<const double *>(v.data());
+void read_range(CTF::Tensor<F> *t, int64_t first, int64_t count, F *out) {
+#if defined(ATRIP_USE_CYCLOPS)
+  std::vector<int64_t> idx((size_t)count);
+  for (int64_t q = 0; q < count; q++) idx[q] = first + q;
+  t->read(count, idx.data(), out);
+#else
+  t->read_range(first, count, out);
+#endif
 }
 
+// The input tensors in atrip_tensor order, read slab by slab by the
+// library (atrip_hip_upload_stream): the host never holds a copy of a
+// large tensor, where the reference slices each one per SliceUnion
+// (src/atrip/Unions.hpp:22-75).
 template <typename F>
-int upload(atrip_hip_ctx *h, const atrip_sys_desc *d, int *sys);
-template <>
-int upload<double>(atrip_hip_ctx *h, const atrip_sys_desc *d, int *sys) {
-  return atrip_hip_upload(h, d, sys);
-}
-template <>
-int upload<Complex>(atrip_hip_ctx *h, const atrip_sys_desc *d, int *sys) {
-  return atrip_hip_upload_complex(h, d, sys);
-}
+struct TensorReader {
+  CTF::Tensor<F> *t[9] = {};
+  std::string error;
+  static int read(void *user, int tensor, int64_t first, int64_t count,
+                  double *out) {
+    auto *r = static_cast<TensorReader *>(user);
+    try {
+      if (tensor < 0 || tensor > 8 || !r->t[tensor])
+        throw std::string("no tensor with id ") + std::to_string(tensor);
+      read_range(r->t[tensor], first, count, reinterpret_cast<F *>(out));
+      return 0;
+    } catch (const std::string &e) {
+      r->error = e;
+    } catch (const std::exception &e) {
+      r->error = e.what();
+    }
+    return 1;
+  }
+};
 
 template <typename F>
 Atrip::Output run_impl(Atrip::Input<F> const &in, size_t np);
@@ -109,39 +126,32 @@ Atrip::Output run_impl(Atrip::Input<F> const &in, size_t np) {
             << "Atrip: Nv: " << Nv << "\n"
             << "Atrip: np: " << np << "\n";
 
-  // host copies in CTF layout (the reference's read_all, :179-181)
-  std::vector<F> ei = read_tensor(in.ei, "ei", No);
-  std::vector<F> ea = read_tensor(in.ea, "ea", Nv);
-  std::vector<F> tai = read_tensor(in.Tph, "Tai", Nv * No);
-  std::vector<F> tabij = read_tensor(in.Tpphh, "Tabij", Nv * Nv * No * No);
-  std::vector<F> vabij = read_tensor(in.Vpphh, "Vabij", Nv * Nv * No * No);
-  std::vector<F> vijka = read_tensor(in.Vhhhp, "Vijka", No * No * No * Nv);
-  std::vector<F> vabci = read_tensor(in.Vppph, "Vabci", Nv * Nv * Nv * No);
-  std::vector<F> jijka, jabci;
   const bool with_j = in.Jhhhp && in.Jppph;
-  if (with_j) {
-    jijka = read_tensor(in.Jhhhp, "Jijka", No * No * No * Nv);
-    jabci = read_tensor(in.Jppph, "Jabci", Nv * Nv * Nv * No);
+  TensorReader<F> reader;
+  CTF::Tensor<F> *ts[9] = {in.ei,    in.ea,    in.Tph,   in.Tpphh, in.Vpphh,
+                           in.Vhhhp, in.Vppph, in.Jhhhp, in.Jppph};
+  const char *names[9] = {"ei",    "ea",    "Tai",   "Tabij", "Vabij",
+                          "Vijka", "Vabci", "Jijka", "Jabci"};
+  const size_t sizes[9] = {No,           Nv,           Nv * No,
+                           Nv * Nv * No * No, Nv * Nv * No * No,
+                           No * No * No * Nv, Nv * Nv * Nv * No,
+                           No * No * No * Nv, Nv * Nv * Nv * No};
+  for (int q = 0; q < (with_j ? 9 : 7); q++) {
+    check_size(ts[q], names[q], sizes[q]);
+    reader.t[q] = ts[q];
   }
 
   Ctx ctx(in.device);
-  atrip_sys_desc d{};
-  d.No = (int64_t)No;
-  d.Nv = (int64_t)Nv;
-  d.ei = raw(ei);
-  d.ea = raw(ea);
-  d.Tai = raw(tai);
-  d.Tabij = raw(tabij);
-  d.Vabij = raw(vabij);
-  d.Vijka = raw(vijka);
-  d.Vabci = raw(vabci);
-  d.Jijka = with_j ? raw(jijka) : nullptr;
-  d.Jabci = with_j ? raw(jabci) : nullptr;
+  ctx.check(atrip_hip_set_option(ctx.h, "stage_kb", in.stage_kb), "stage_kb");
   int sys = 0;
-  ctx.check(upload<F>(ctx.h, &d, &sys), "upload");
-  // host copies are not needed once resident
-  std::vector<F>().swap(vabci);
-  std::vector<F>().swap(jabci);
+  const int flags = (with_j ? ATRIP_STREAM_WITH_J : 0) |
+                    (std::is_same<F, Complex>::value ? ATRIP_STREAM_COMPLEX : 0);
+  if (atrip_hip_upload_stream(ctx.h, (int64_t)No, (int64_t)Nv, flags,
+                              &TensorReader<F>::read, &reader, &sys) != 0) {
+    std::string why = atrip_hip_last_error(ctx.h);
+    if (!reader.error.empty()) why += " (" + reader.error + ")";
+    throw std::string("atrip: upload: ") + why;
+  }
   if (in.delete_Vppph) delete in.Vppph;  // reference: src/atrip/Atrip.cxx:310
 
   const uint64_t n = (uint64_t)Nv * (Nv + 1) * (Nv + 2) / 6 - Nv;

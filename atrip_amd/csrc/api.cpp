@@ -255,29 +255,118 @@ void alloc_system(atrip_hip_ctx *ctx, DevSystem &s, int64_t No, int64_t Nv,
   (void)ctx;
 }
 
-// Stage a host tensor through a bounded device buffer, `slab` elements per
-// step of its slowest CTF index (n_slow steps), and hand each staged chunk
-// to `place(dev_chunk, first_slow, count_slow)`.
+// Where the values of a system's input tensors come from.  Values are in
+// CTF linear (column-major) order; a value is `comp` doubles (1 real, 2 an
+// interleaved (re, im) pair).  fetch() copies values [first, first + count)
+// of tensor t (enum atrip_tensor) to device memory on stream st; host()
+// reads them into host memory (the small tensors).
+struct Source {
+  int comp = 1;
+  virtual ~Source() = default;
+  virtual void fetch(int t, int64_t first, int64_t count, double *dst,
+                     hipStream_t st) = 0;
+  virtual void host(int t, int64_t first, int64_t count, double *out) = 0;
+};
+
+// Tensors in caller memory (the pointers of an atrip_sys_desc).
+struct DescSource : Source {
+  const double *p[9];
+  DescSource(const atrip_sys_desc *d, int comp_) {
+    comp = comp_;
+    const double *q[9] = {d->ei,    d->ea,    d->Tai,   d->Tabij, d->Vabij,
+                          d->Vijka, d->Vabci, d->Jijka, d->Jabci};
+    std::copy(q, q + 9, p);
+  }
+  void fetch(int t, int64_t first, int64_t count, double *dst,
+             hipStream_t st) override {
+    check(hipMemcpyAsync(dst, p[t] + first * comp,
+                         (size_t)(count * comp) * sizeof(double),
+                         hipMemcpyHostToDevice, st),
+          "upload chunk");
+  }
+  void host(int t, int64_t first, int64_t count, double *out) override {
+    std::memcpy(out, p[t] + first * comp, (size_t)(count * comp) * sizeof(double));
+  }
+};
+
+// Tensors read through the caller's reader, piece by piece into a pinned
+// host buffer of bounded size (two halves: one is read into while the copy
+// of the other is in flight), so the host holds no tensor whole.
+struct ReaderSource : Source {
+  atrip_read_fn fn;
+  void *user;
+  double *pinned = nullptr;
+  int64_t half = 0;  // doubles per half
+  hipEvent_t done[2] = {nullptr, nullptr};
+  bool recorded[2] = {false, false};
+  ReaderSource(atrip_read_fn f, void *u, int comp_, int64_t cap_bytes)
+      : fn(f), user(u) {
+    comp = comp_;
+    half = std::max<int64_t>(comp, cap_bytes / (2 * (int64_t)sizeof(double)));
+    half = half / comp * comp;
+    check(hipHostMalloc((void **)&pinned, (size_t)(2 * half) * sizeof(double),
+                        hipHostMallocDefault),
+          "alloc pinned staging");
+    for (auto &e : done) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+  }
+  ~ReaderSource() override {
+    for (int h = 0; h < 2; h++)
+      if (recorded[h]) (void)hipEventSynchronize(done[h]);  // copies done
+    for (auto e : done)
+      if (e) (void)hipEventDestroy(e);
+    if (pinned) (void)hipHostFree(pinned);
+  }
+  void read(int t, int64_t first, int64_t count, double *out) {
+    if (fn(user, t, first, count, out) != 0)
+      throw Fail{"tensor reader failed (tensor " + std::to_string(t) +
+                 ", values " + std::to_string(first) + "+" +
+                 std::to_string(count) + ")"};
+  }
+  void fetch(int t, int64_t first, int64_t count, double *dst,
+             hipStream_t st) override {
+    const int64_t per = half / comp;  // values per piece
+    int h = 0;
+    for (int64_t off = 0; off < count; off += per, h ^= 1) {
+      const int64_t n = std::min(per, count - off);
+      double *buf = pinned + h * half;
+      if (recorded[h]) check(hipEventSynchronize(done[h]), "pinned staging");
+      read(t, first + off, n, buf);
+      check(hipMemcpyAsync(dst + off * comp, buf,
+                           (size_t)(n * comp) * sizeof(double),
+                           hipMemcpyHostToDevice, st),
+            "upload chunk");
+      check(hipEventRecord(done[h], st), "event");
+      recorded[h] = true;
+    }
+  }
+  void host(int t, int64_t first, int64_t count, double *out) override {
+    read(t, first, count, out);
+  }
+};
+
+// Stage tensor t through a bounded device buffer (the context's stage_kb),
+// whole slabs of `slab` values of its slowest CTF index per step (n_slow
+// slabs; at least one slab per step), and hand each staged chunk to
+// `place(dev_chunk, first_slow, count_slow)`.
 template <typename P>
-void staged(atrip_hip_ctx *ctx, const double *host, int64_t slab,
+void staged(atrip_hip_ctx *ctx, Source &src, int t, int64_t slab,
             int64_t n_slow, P &&place) {
-  const int64_t budget = (int64_t)1 << 29;  // 4 GiB of doubles
+  const int64_t budget = std::max<int64_t>(
+      1, ctx->stage_kb * 1024 / (int64_t)(sizeof(double) * src.comp));
   int64_t per = std::max<int64_t>(1, budget / std::max<int64_t>(slab, 1));
   per = std::min(per, n_slow);
   void *stage = nullptr;
-  check(hipMalloc(&stage, (size_t)(per * slab) * sizeof(double)),
+  check(hipMalloc(&stage, (size_t)(per * slab * src.comp) * sizeof(double)),
         "alloc staging");
   try {
     for (int64_t s0 = 0; s0 < n_slow; s0 += per) {
       const int64_t cnt = std::min(per, n_slow - s0);
-      check(hipMemcpyAsync(stage, host + s0 * slab,
-                           (size_t)(cnt * slab) * sizeof(double),
-                           hipMemcpyHostToDevice, ctx->stream),
-            "upload chunk");
+      src.fetch(t, s0 * slab, cnt * slab, (double *)stage, ctx->stream);
       place((const double *)stage, s0, cnt);
       check(hipStreamSynchronize(ctx->stream), "relayout");
     }
   } catch (...) {
+    (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(stage);
     throw;
   }
@@ -332,77 +421,105 @@ struct TempOperands {
   }
 };
 
-void upload_host(atrip_hip_ctx *ctx, DevSystem &s, const atrip_sys_desc *d) {
+// Re-lays out every input tensor of `src` into the device arrays of s (and
+// of im, the imaginary planes, for a complex source: src.comp == 2).  Each
+// large tensor is streamed in slabs of its slowest CTF index through the
+// staging buffer; every transpose reads one part of the interleaved values
+// (stride comp).
+void upload_tensors(atrip_hip_ctx *ctx, DevSystem &s, DevSystem *im,
+                    Source &src) {
   const int64_t No = s.No, Nv = s.Nv, N2 = No * No, Nv2 = Nv * Nv;
   const int64_t Kv = s.Kv, Ko = s.Ko;
+  const int m = src.comp;
+  if ((m == 2) != (im != nullptr)) throw Fail{"internal: complex parts"};
+  DevSystem *dst[2] = {&s, im};
   hipStream_t st = ctx->stream;
   TempOperands tmp{s, st, {}, {}};
   tmp.add(s.T3, s.sT3, s.tT3, tri_geom_a(No, Nv, Kv));
   tmp.add(s.T2N, s.sT2N, s.tT2N, tri_geom_b(No, Nv, Ko));
   tmp.add(s.T2TN, s.sT2TN, s.tT2TN, tri_geom_b(No, Nv, Ko));
   // Tabij {Nv,Nv,No,No}: slowest index j
-  staged(ctx, d->Tabij, Nv2 * No, No,
+  staged(ctx, src, ATRIP_TENSOR_TABIJ, Nv2 * No, No,
          [&](const double *c, int64_t j0, int64_t cj) {
-           // T3[x][p + No q][E] = T[x,E,p,q], rows of this chunk: q = j
-           check(launch_transpose(c, s.T3 + j0 * No * Kv, No * cj, Nv, Nv,
-                                  Nv2, Nv, 1, Kv, 1, N2 * Kv, 1.0, st),
-                 "relayout T3");
-           // T2N[pair][n][L] = -T[pair,L,n], n = j
-           check(launch_transpose(c, s.T2N + j0 * Ko, cj, No, Nv2, Nv2 * No,
-                                  Nv2, 1, Ko, 1, No * Ko, -1.0, st),
-                 "relayout T2N");
-           // T2TN[pair][n][L] = -T[pair,n,L], L = j
-           check(launch_transpose(c, s.T2TN + j0, cj, No, Nv2, Nv2 * No, Nv2,
-                                  1, 1, Ko, No * Ko, -1.0, st),
-                 "relayout T2TN");
+           for (int part = 0; part < m; part++) {
+             DevSystem &d = *dst[part];
+             // T3[x][p + No q][E] = T[x,E,p,q], rows of this chunk: q = j
+             check(launch_transpose(c + part, d.T3 + j0 * No * Kv, No * cj, Nv,
+                                    Nv, m * Nv2, m * Nv, m, Kv, 1, N2 * Kv,
+                                    1.0, st),
+                   "relayout T3");
+             // T2N[pair][n][L] = -T[pair,L,n], n = j
+             check(launch_transpose(c + part, d.T2N + j0 * Ko, cj, No, Nv2,
+                                    m * Nv2 * No, m * Nv2, m, Ko, 1, No * Ko,
+                                    -1.0, st),
+                   "relayout T2N");
+             // T2TN[pair][n][L] = -T[pair,n,L], L = j
+             check(launch_transpose(c + part, d.T2TN + j0, cj, No, Nv2,
+                                    m * Nv2 * No, m * Nv2, m, 1, Ko, No * Ko,
+                                    -1.0, st),
+                   "relayout T2TN");
+           }
          });
   // Vabij {Nv,Nv,No,No}: VHH[pair][i + No j]
-  staged(ctx, d->Vabij, Nv2 * No, No,
+  staged(ctx, src, ATRIP_TENSOR_VABIJ, Nv2 * No, No,
          [&](const double *c, int64_t j0, int64_t cj) {
-           check(launch_transpose(c, s.VHH + j0 * No, 1, No * cj, Nv2, 0, Nv2,
-                                  1, 0, 1, N2, 1.0, st),
-                 "relayout VHH");
+           for (int part = 0; part < m; part++)
+             check(launch_transpose(c + part, dst[part]->VHH + j0 * No, 1,
+                                    No * cj, Nv2, 0, m * Nv2, m, 0, 1, N2, 1.0,
+                                    st),
+                   "relayout VHH");
          });
   // Vijka {No,No,No,Nv}: H3[x][row][L] = V[row,L,x]
-  auto hhh = [&](const double *host, double *dst) {
-    staged(ctx, host, N2 * No, Nv,
+  auto hhh = [&](int t, double *DevSystem::*arr) {
+    staged(ctx, src, t, N2 * No, Nv,
            [&](const double *c, int64_t x0, int64_t cx) {
-             check(launch_transpose(c, dst + x0 * N2 * Ko, cx, No, N2,
-                                    N2 * No, N2, 1, N2 * Ko, 1, Ko, 1.0, st),
-                   "relayout H3");
+             for (int part = 0; part < m; part++)
+               check(launch_transpose(c + part, dst[part]->*arr + x0 * N2 * Ko,
+                                      cx, No, N2, m * N2 * No, m * N2, m,
+                                      N2 * Ko, 1, Ko, 1.0, st),
+                     "relayout H3");
            });
   };
   // Vabci {Nv,Nv,Nv,No}: V3[pair][r][E] = V[pair,E,r]
-  auto ppph = [&](const double *host, double *dst) {
-    staged(ctx, host, Nv2 * Nv, No,
+  auto ppph = [&](int t, double *DevSystem::*arr) {
+    staged(ctx, src, t, Nv2 * Nv, No,
            [&](const double *c, int64_t r0, int64_t cr) {
-             check(launch_transpose(c, dst + r0 * Kv, cr, Nv, Nv2, Nv2 * Nv,
-                                    Nv2, 1, Kv, 1, No * Kv, 1.0, st),
-                   "relayout V3");
+             for (int part = 0; part < m; part++)
+               check(launch_transpose(c + part, dst[part]->*arr + r0 * Kv, cr,
+                                      Nv, Nv2, m * Nv2 * Nv, m * Nv2, m, Kv, 1,
+                                      No * Kv, 1.0, st),
+                     "relayout V3");
            });
   };
   tmp.convert();
   tmp.add(s.H3, s.sH3, s.tH3, tri_geom_a(No, Nv, Ko));
-  hhh(d->Vijka, s.H3);
+  hhh(ATRIP_TENSOR_VIJKA, &DevSystem::H3);
   tmp.convert();
   tmp.add(s.V3, s.sV3, s.tV3, tri_geom_b(No, Nv, Kv));
-  ppph(d->Vabci, s.V3);
+  ppph(ATRIP_TENSOR_VABCI, &DevSystem::V3);
   tmp.convert();
-  if (d->Jijka) {
+  if (s.has_j) {
     tmp.add(s.H3J, s.sH3J, s.tH3J, tri_geom_a(No, Nv, Ko));
-    hhh(d->Jijka, s.H3J);
+    hhh(ATRIP_TENSOR_JIJKA, &DevSystem::H3J);
     tmp.convert();
     tmp.add(s.V3J, s.sV3J, s.tV3J, tri_geom_b(No, Nv, Kv));
-    ppph(d->Jabci, s.V3J);
+    ppph(ATRIP_TENSOR_JABCI, &DevSystem::V3J);
     tmp.convert();
   }
-  check(hipMemcpy(s.Tai, d->Tai, sizeof(double) * Nv * No,
-                  hipMemcpyHostToDevice),
-        "upload Tai");
-  check(hipMemcpy(s.ei, d->ei, sizeof(double) * No, hipMemcpyHostToDevice),
-        "upload ei");
-  check(hipMemcpy(s.ea, d->ea, sizeof(double) * Nv, hipMemcpyHostToDevice),
-        "upload ea");
+  // the small tensors whole (the reference's read_all, Atrip.cxx:176-181)
+  auto small = [&](int t, int64_t n, double *DevSystem::*arr, const char *what) {
+    std::vector<double> buf((size_t)(n * m)), part((size_t)n);
+    src.host(t, 0, n, buf.data());
+    for (int q = 0; q < m; q++) {
+      for (int64_t e = 0; e < n; e++) part[e] = buf[e * m + q];
+      check(hipMemcpy(dst[q]->*arr, part.data(), (size_t)n * sizeof(double),
+                      hipMemcpyHostToDevice),
+            what);
+    }
+  };
+  small(ATRIP_TENSOR_TAI, Nv * No, &DevSystem::Tai, "upload Tai");
+  small(ATRIP_TENSOR_EI, No, &DevSystem::ei, "upload ei");
+  small(ATRIP_TENSOR_EA, Nv, &DevSystem::ea, "upload ea");
 }
 
 // The imaginary planes of a complex system in the slots of the real ones
@@ -824,7 +941,8 @@ int atrip_hip_upload(atrip_hip_ctx *ctx, const atrip_sys_desc *desc,
     DevSystem s;
     try {
       alloc_system(ctx, s, desc->No, desc->Nv, desc->Jijka != nullptr);
-      upload_host(ctx, s, desc);
+      DescSource src(desc, 1);
+      upload_tensors(ctx, s, nullptr, src);
     } catch (...) {
       free_system(s);
       throw;
@@ -837,13 +955,22 @@ int atrip_hip_upload(atrip_hip_ctx *ctx, const atrip_sys_desc *desc,
 
 int atrip_hip_upload_synthetic(atrip_hip_ctx *ctx, int64_t No, int64_t Nv,
                                uint64_t seed, int with_j, int *sys) {
+  return atrip_hip_upload_synthetic_scaled(ctx, No, Nv, seed, with_j, 1.0,
+                                           sys);
+}
+
+int atrip_hip_upload_synthetic_scaled(atrip_hip_ctx *ctx, int64_t No,
+                                      int64_t Nv, uint64_t seed, int with_j,
+                                      double scale, int *sys) {
   return guarded(ctx, [&] {
     if (!sys) throw Fail{"null argument"};
+    if (!std::isfinite(scale)) throw Fail{"scale must be finite"};
     check(hipSetDevice(ctx->device), "hipSetDevice");
     DevSystem s;
     try {
       alloc_system(ctx, s, No, Nv, with_j != 0);
-      check(launch_synth(s, seed, with_j != 0, ctx->stream), "synthetic fill");
+      check(launch_synth(s, seed, with_j != 0, ctx->stream, 0, scale),
+            "synthetic fill");
       check(hipStreamSynchronize(ctx->stream), "synthetic fill");
     } catch (...) {
       free_system(s);
@@ -865,39 +992,48 @@ int atrip_hip_upload_complex(atrip_hip_ctx *ctx, const atrip_sys_desc *desc,
     if ((desc->Jijka == nullptr) != (desc->Jabci == nullptr))
       throw Fail{"Jijka and Jabci must be given together"};
     check(hipSetDevice(ctx->device), "hipSetDevice");
-    const int64_t No = desc->No, Nv = desc->Nv;
-    const bool with_j = desc->Jijka != nullptr;
     DevSystem s;
     try {
-      alloc_system(ctx, s, No, Nv, with_j, true);
-      // real, then imaginary parts through the real upload path
-      const int64_t N2 = No * No, Nv2 = Nv * Nv;
-      const double *src[9] = {desc->ei,    desc->ea,    desc->Tai,
-                              desc->Tabij, desc->Vabij, desc->Vijka,
-                              desc->Vabci, desc->Jijka, desc->Jabci};
-      const int64_t len[9] = {No,         Nv,        Nv * No,
-                              Nv2 * N2,   Nv2 * N2,  N2 * No * Nv,
-                              Nv2 * Nv * No, N2 * No * Nv, Nv2 * Nv * No};
-      for (int part = 0; part < 2; part++) {
-        std::vector<std::vector<double>> host(9);
-        for (int t = 0; t < 9; t++) {
-          if (!src[t]) continue;
-          host[t].resize(len[t]);
-          for (int64_t q = 0; q < len[t]; q++) host[t][q] = src[t][2 * q + part];
-        }
-        atrip_sys_desc d{No, Nv, host[0].data(), host[1].data(), host[2].data(),
-                         host[3].data(), host[4].data(), host[5].data(),
-                         host[6].data(), with_j ? host[7].data() : nullptr,
-                         with_j ? host[8].data() : nullptr};
-        if (part == 0) {
-          upload_host(ctx, s, &d);
-        } else {
-          DevSystem v = imag_view(s);
-          upload_host(ctx, v, &d);
-        }
-      }
+      alloc_system(ctx, s, desc->No, desc->Nv, desc->Jijka != nullptr, true);
+      // real and imaginary planes from the interleaved values in one pass
+      DescSource src(desc, 2);
+      DevSystem im = imag_view(s);
+      upload_tensors(ctx, s, &im, src);
       complex_negations(ctx, s);
     } catch (...) {
+      free_system(s);
+      throw;
+    }
+    const int id = ctx->next_sys++;
+    ctx->systems.emplace(id, std::move(s));
+    *sys = id;
+  });
+}
+
+int atrip_hip_upload_stream(atrip_hip_ctx *ctx, int64_t No, int64_t Nv,
+                            int flags, atrip_read_fn read, void *user,
+                            int *sys) {
+  return guarded(ctx, [&] {
+    if (!read || !sys) throw Fail{"null argument"};
+    if (flags & ~(ATRIP_STREAM_WITH_J | ATRIP_STREAM_COMPLEX))
+      throw Fail{"unknown upload flags"};
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    const bool with_j = flags & ATRIP_STREAM_WITH_J;
+    const bool cplx = flags & ATRIP_STREAM_COMPLEX;
+    DevSystem s;
+    try {
+      alloc_system(ctx, s, No, Nv, with_j, cplx);
+      ReaderSource src(read, user, cplx ? 2 : 1,
+                       std::min<int64_t>(ctx->stage_kb * 1024, PINNED_MAX));
+      if (cplx) {
+        DevSystem im = imag_view(s);
+        upload_tensors(ctx, s, &im, src);
+        complex_negations(ctx, s);
+      } else {
+        upload_tensors(ctx, s, nullptr, src);
+      }
+    } catch (...) {
+      (void)hipStreamSynchronize(ctx->stream);
       free_system(s);
       throw;
     }
@@ -1012,6 +1148,10 @@ int atrip_hip_set_option(atrip_hip_ctx *ctx, const char *key, int64_t value) {
       if (value < 0 || value > 3)
         throw Fail{"operand_store must be 0, 1, 2 or 3"};
       ctx->operand_store = (int)value;
+    } else if (k == "stage_kb") {
+      if (value < 1 || value > ((int64_t)1 << 30))
+        throw Fail{"stage_kb must be in 1..2^30"};
+      ctx->stage_kb = value;
     } else if (k == "batch_scratch_mb") {
       if (value < 16 || value > (1 << 18))
         throw Fail{"batch_scratch_mb must be in 16..262144"};
@@ -1046,6 +1186,7 @@ int atrip_hip_get_option(atrip_hip_ctx *ctx, const char *key, int64_t *value) {
     else if (k == "timing") *value = ctx->timing;
     else if (k == "tile") *value = ctx->tile;
     else if (k == "operand_store") *value = ctx->operand_store;
+    else if (k == "stage_kb") *value = ctx->stage_kb;
     else if (k == "batch_scratch_mb") *value = ctx->batch_scratch_mb;
     else if (k == "batch_grid") *value = ctx->batch_grid;
     else if (k == "batch_order") *value = ctx->batch_order;

@@ -39,7 +39,7 @@ __device__ inline double re_div(cplx a, cplx b) {
 
 struct EnergyZParams {
   const double *Wr, *Wi, *WJr, *WJi;  // cubes, [slot][W3]
-  const double *Tair, *Taii, *VHHr, *VHHi, *eir, *eii, *ear, *eai;
+  const double *Tair, *Taii, *VHHr, *VHHi, *eir, *eii, *ear;
   const int32_t *windows;
   const uint64_t *aprefix;
   double *e_slot;
@@ -86,8 +86,10 @@ __global__ __launch_bounds__(256) void energy_z_kernel(EnergyZParams P) {
   int distinct = 0;
   if (a == b) distinct++;
   if (b == c) distinct--;
-  const cplx epsabc = (cplx{P.ear[a], P.eai[a]} + cplx{P.ear[b], P.eai[b]}) +
-                      cplx{P.ear[c], P.eai[c]};
+  // the reference keeps only the real part of the virtual-orbital sum
+  // (std::real, src/atrip/Atrip.cxx:643-644); the occupied energies of the
+  // denominator stay complex
+  const cplx epsabc = cplx{(P.ear[a] + P.ear[b]) + P.ear[c], 0.0};
   const int nbk = (int)P.nbk, no = (int)No, tid = threadIdx.x;
   auto tai = [&](int64_t x, int occ) {
     return cplx{P.tai_sign * P.Tair[x + Nv * occ], P.tai_sign * P.Taii[x + Nv * occ]};
@@ -214,7 +216,6 @@ hipError_t launch_energy_z(const DevSystem &s, const BatchArgs &b,
   P.eir = s.ei;
   P.eii = s.eii;
   P.ear = s.ea;
-  P.eai = s.eai;
   P.windows = b.windows;
   P.aprefix = s.aprefix;
   P.e_slot = e_slot;

@@ -34,6 +34,10 @@ int guarded(atrip_hip_ctx *ctx, F &&f) {
     ctx->err = x.what();
   }
   if (ctx->err.empty()) ctx->err = "unknown failure";
+  // HIP keeps the error of a failed runtime call (an allocation that found
+  // no memory, say) until it is read; read it here so that the next launch
+  // check on this thread does not report it again.
+  (void)hipGetLastError();
   return 2;
 }
 

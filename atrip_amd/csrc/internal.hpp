@@ -13,6 +13,9 @@
 
 namespace atrip_amd {
 
+// Largest pinned host buffer of a reader upload (bytes).
+constexpr int64_t PINNED_MAX = (int64_t)256 << 20;
+
 // Contraction-index padding of the device layouts: every K run of the
 // doubles GEMMs is a multiple of the 16-element K chunk of the kernels.
 constexpr int KCHUNK = 16;
@@ -200,6 +203,7 @@ struct atrip_hip_ctx {
   int timing = 0;
   int tile = -1;  // -1: chosen per system shape
   int operand_store = 0;  // store mode of the next uploads (DevSystem::store)
+  int64_t stage_kb = 4 << 20;  // device staging of host uploads (KiB)
   // scratch, grown on demand
   double *G = nullptr;       // class buffers [ncls][slots][No^3]
   size_t G_bytes = 0;
@@ -290,8 +294,11 @@ hipError_t launch_transpose(const double *src, double *dst, int64_t n_outer,
 // Generate a synthetic system in place; tensor_flag is or-ed into every
 // tensor id (ATRIP_SYN_IMAG fills the imaginary parts of a complex system,
 // given a DevSystem whose arrays are its imaginary planes).
+// Every amplitude and integral value is multiplied by `scale` (orbital
+// energies are not).
 hipError_t launch_synth(DevSystem &s, uint64_t seed, bool with_j,
-                        hipStream_t st, int tensor_flag = 0);
+                        hipStream_t st, int tensor_flag = 0,
+                        double scale = 1.0);
 // dst[q] = scale * src[q], q < n.
 hipError_t launch_scale(const double *src, double *dst, int64_t n,
                         double scale, hipStream_t st);

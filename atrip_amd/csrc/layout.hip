@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void synth_kernel(double *dst, float *sh,
                                                     int target, int tensor,
                                                     uint64_t stream, int64_t No,
                                                     int64_t Nv, int64_t Kv,
-                                                    int64_t Ko) {
+                                                    int64_t Ko, double scale) {
   const int64_t N2 = No * No, Nv2 = Nv * Nv;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void synth_kernel(double *dst, float *sh,
     double v = 0.0;
     if (idx >= 0)
       v = sign * atrip_synth_value(stream, tensor, (uint64_t)No, (uint64_t)Nv,
-                                   (uint64_t)idx);
+                                   (uint64_t)idx) * scale;
     if (dst) dst[q] = v;
     if (sh) split_store(v, sh, sl, q);
     if (tri.p) tri_store(v, tri, tg, No, q);
@@ -107,14 +107,17 @@ __global__ __launch_bounds__(256) void synth_kernel(double *dst, float *sh,
 hipError_t synth_fill(double *dst, const DevSystem::Split &o, const TriOp &tri,
                       const TriGeom &tg, int64_t n, SynthTarget target,
                       int tensor, uint64_t seed, const DevSystem &s,
-                      hipStream_t st) {
+                      double scale, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const uint64_t stream = atrip_synth_stream(seed, tensor);
+  // amplitudes and integrals carry the scale, orbital energies do not
+  const int base = tensor & 15;
+  if (base == ATRIP_SYN_EI || base == ATRIP_SYN_EA) scale = 1.0;
   int64_t blocks = (n + 255) / 256;
   if (blocks > 65536L * 16) blocks = 65536L * 16;
   hipLaunchKernelGGL(synth_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
                      dst, o.h, o.l, tri, tg, n, (int)target, tensor, stream, s.No,
-                     s.Nv, s.Kv, s.Ko);
+                     s.Nv, s.Kv, s.Ko, scale);
   return hipGetLastError();
 }
 
@@ -142,7 +145,7 @@ hipError_t launch_transpose(const double *src, double *dst, int64_t n_outer,
 }
 
 hipError_t launch_synth(DevSystem &s, uint64_t seed, bool with_j,
-                        hipStream_t st, int tensor_flag) {
+                        hipStream_t st, int tensor_flag, double scale) {
   const int64_t No = s.No, Nv = s.Nv, N2 = No * No, Nv2 = Nv * Nv;
   hipError_t e;
   const DevSystem::Split none;
@@ -155,12 +158,12 @@ hipError_t launch_synth(DevSystem &s, uint64_t seed, bool with_j,
   if ((e = synth_fill(s.store == 0 ? ptr : nullptr,                         \
                       s.store == 1 || s.store == 2 ? sp : none,             \
                       s.store == 3 ? tp : tnone, g, (g).no * (g).nr * (g).K, \
-                      tgt, (tensor) | tensor_flag, seed, s, st)) !=         \
+                      tgt, (tensor) | tensor_flag, seed, s, scale, st)) !=  \
       hipSuccess)                                                           \
     return e;
 #define SYN64(ptr, n, tgt, tensor)                                         \
   if ((e = synth_fill(ptr, none, tnone, ga, n, tgt, (tensor) | tensor_flag, \
-                      seed, s, st)) != hipSuccess)                          \
+                      seed, s, scale, st)) != hipSuccess)                   \
     return e;
   SYN(s.T3, s.sT3, s.tT3, ga, S_T3, ATRIP_SYN_TABIJ);
   SYN(s.V3, s.sV3, s.tV3, gb, S_V3, ATRIP_SYN_VABCI);

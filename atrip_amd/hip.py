@@ -25,6 +25,7 @@ PRECISIONS = {"fp64": FP64, "split_fp32": SPLIT_FP32,
 SYMBOLS = (
     "atrip_hip_create", "atrip_hip_destroy", "atrip_hip_last_error",
     "atrip_hip_upload", "atrip_hip_upload_synthetic", "atrip_hip_release",
+    "atrip_hip_upload_synthetic_scaled", "atrip_hip_upload_stream",
     "atrip_hip_upload_complex", "atrip_hip_upload_synthetic_complex",
     "atrip_hip_run", "atrip_hip_tuple_energies", "atrip_hip_set_option",
     "atrip_hip_get_option", "atrip_hip_kernel_stats", "atrip_hip_reset_stats",
@@ -37,6 +38,14 @@ SYMBOLS = (
 )
 
 TENSORS = ("ei", "ea", "Tai", "Tabij", "Vabij", "Vijka", "Vabci", "Jijka", "Jabci")
+
+
+STREAM_WITH_J = 1
+STREAM_COMPLEX = 2
+
+# int (*)(void *user, int tensor, int64_t first, int64_t count, double *out)
+READ_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_int64,
+                      C.POINTER(C.c_double))
 
 
 class HipError(RuntimeError):
@@ -70,6 +79,12 @@ def load():
     L.atrip_hip_upload.argtypes = [vp, C.POINTER(SysDesc), C.POINTER(C.c_int)]
     L.atrip_hip_upload_synthetic.argtypes = [
         vp, C.c_int64, C.c_int64, C.c_uint64, C.c_int, C.POINTER(C.c_int)]
+    L.atrip_hip_upload_synthetic_scaled.argtypes = [
+        vp, C.c_int64, C.c_int64, C.c_uint64, C.c_int, C.c_double,
+        C.POINTER(C.c_int)]
+    L.atrip_hip_upload_stream.argtypes = [
+        vp, C.c_int64, C.c_int64, C.c_int, READ_FN, C.c_void_p,
+        C.POINTER(C.c_int)]
     L.atrip_hip_release.argtypes = [vp, C.c_int]
     L.atrip_hip_upload_complex.argtypes = [vp, C.POINTER(SysDesc), C.POINTER(C.c_int)]
     L.atrip_hip_upload_synthetic_complex.argtypes = [
@@ -203,6 +218,42 @@ class Context:
         sys_ = C.c_int()
         self._check(self._L.atrip_hip_upload_synthetic(
             self._h, no, nv, seed, int(with_j), C.byref(sys_)), "upload_synthetic")
+        return sys_.value
+
+    def upload_synthetic_scaled(self, no: int, nv: int, seed: int, scale: float,
+                                with_j=False) -> int:
+        """Synthetic system with amplitudes and integrals times `scale`."""
+        sys_ = C.c_int()
+        self._check(self._L.atrip_hip_upload_synthetic_scaled(
+            self._h, no, nv, seed, int(with_j), float(scale), C.byref(sys_)),
+            "upload_synthetic_scaled")
+        return sys_.value
+
+    def upload_stream(self, no: int, nv: int, read, with_j=False, complex_=False) -> int:
+        """Upload through a reader: read(tensor_name, first, count) returns the
+        values [first, first + count) of that tensor in CTF order (float64, or
+        complex128 for complex systems); the library asks for slabs."""
+        comp = 2 if complex_ else 1
+
+        def cb(_user, tensor, first, count, out):
+            try:
+                vals = np.asarray(read(TENSORS[tensor], first, count))
+                if complex_:
+                    vals = np.ascontiguousarray(vals, dtype=np.complex128).view(np.float64)
+                else:
+                    vals = np.ascontiguousarray(vals, dtype=np.float64)
+                if vals.size != count * comp:
+                    return 1
+                np.ctypeslib.as_array(out, shape=(count * comp,))[:] = vals
+                return 0
+            except Exception:  # reported through the library's status
+                return 1
+
+        fn = READ_FN(cb)
+        flags = (STREAM_WITH_J if with_j else 0) | (STREAM_COMPLEX if complex_ else 0)
+        sys_ = C.c_int()
+        self._check(self._L.atrip_hip_upload_stream(
+            self._h, no, nv, flags, fn, None, C.byref(sys_)), "upload_stream")
         return sys_.value
 
     def upload_complex(self, no: int, nv: int, t: dict) -> int:

@@ -128,6 +128,9 @@ struct Atrip {
     // MFMA plus hi*lo + lo*hi on bf16 MFMA) or "fp32" (hi*hi only); FP32
     // accumulation and FP64 energy in every form
     ATRIP_ADD_ATTRIBUTE(std::string, mixed_form, "bf16x3")
+    // Upload staging (KiB): device chunk of each large tensor and bound of
+    // the pinned host buffer its slabs are read through (at most 256 MiB)
+    ATRIP_ADD_ATTRIBUTE(int64_t, stage_kb, (int64_t)4 << 20)
   };
 
   struct Output {

@@ -1,8 +1,9 @@
 // Minimal stand-in for the part of Cyclops CTF that the (T) interface uses,
 // for builds without Cyclops.  With Cyclops, define ATRIP_USE_CYCLOPS and the
-// real <ctf.hpp> is used instead; only `order`, `lens` and
-// `read_all(F*)` are touched by atrip::Atrip::run (as in the reference,
-// src/atrip/Atrip.cxx:72-73 and :179-181).
+// real <ctf.hpp> is used instead.  atrip::Atrip::run touches `order`,
+// `lens` (as in the reference, src/atrip/Atrip.cxx:72-73), `read_all` for
+// the small tensors (:179-181) and ranged reads of the large ones: here
+// `read_range`, with Cyclops its Tensor::read(npair, global_idx, data).
 //
 // Storage is a dense, column-major (first index fastest) host array: the
 // layout CTF's read_all returns.
@@ -67,6 +68,22 @@ class Tensor {
   void read_all(F *out, bool /*unpack*/ = true) const {
     std::memcpy(out, values_.data(), sizeof(F) * values_.size());
   }
+  // Elements with global (column-major) indices global_idx[0..npair) into
+  // data, the form of CTF's Tensor::read(npair, global_idx, data).
+  void read(int64_t npair, const int64_t *global_idx, F *data) const {
+    for (int64_t q = 0; q < npair; q++) data[q] = values_[global_idx[q]];
+  }
+  // Elements [first, first + count) in column-major order: the ranged read
+  // the slab-wise upload asks for (a contiguous global index range).
+  void read_range(int64_t first, int64_t count, F *out) const {
+    if (first < 0 || count < 0 || first + count > size())
+      throw std::string("read_range out of bounds");
+    std::memcpy(out, values_.data() + first, sizeof(F) * count);
+    ++range_reads;
+    if (count > max_range_read) max_range_read = count;
+  }
+  // Instrumentation of read_range: calls and the largest count asked for.
+  mutable int64_t range_reads = 0, max_range_read = 0;
   // Fill from a column-major buffer (counterpart of CTF's write).
   void write_all(const F *in) {
     std::memcpy(values_.data(), in, sizeof(F) * values_.size());

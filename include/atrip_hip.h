@@ -77,6 +77,57 @@ int atrip_hip_upload(atrip_hip_ctx *ctx, const atrip_sys_desc *desc,
 int atrip_hip_upload_synthetic(atrip_hip_ctx *ctx, int64_t No, int64_t Nv,
                                uint64_t seed, int with_j, int *sys);
 
+/* The synthetic system with every amplitude and integral (Tai, Tabij,
+ * Vabij, Vijka, Vabci and the J tensors) multiplied by `scale`; orbital
+ * energies unchanged.  E(T) then scales by scale^4 (each tuple energy is a
+ * product of a doubles or singles term, both bilinear in the amplitudes and
+ * integrals, with W).  Used to state absolute energy errors at molecular
+ * magnitudes on the synthetic shapes.  No counterpart in the reference. */
+int atrip_hip_upload_synthetic_scaled(atrip_hip_ctx *ctx, int64_t No,
+                                      int64_t Nv, uint64_t seed, int with_j,
+                                      double scale, int *sys);
+
+/* Tensor ids of the streaming upload (the order of atrip_sys_desc). */
+enum atrip_tensor {
+  ATRIP_TENSOR_EI = 0,
+  ATRIP_TENSOR_EA = 1,
+  ATRIP_TENSOR_TAI = 2,
+  ATRIP_TENSOR_TABIJ = 3,
+  ATRIP_TENSOR_VABIJ = 4,
+  ATRIP_TENSOR_VIJKA = 5,
+  ATRIP_TENSOR_VABCI = 6,
+  ATRIP_TENSOR_JIJKA = 7,
+  ATRIP_TENSOR_JABCI = 8
+};
+
+/* Reader of the streaming upload: copy the `count` values with CTF linear
+ * (column-major, first index fastest) indices [first, first + count) of
+ * tensor `tensor` (enum atrip_tensor) into `out`; for complex systems each
+ * value is an interleaved (re, im) pair of doubles.  Return 0 on success;
+ * anything else aborts the upload with an error. */
+typedef int (*atrip_read_fn)(void *user, int tensor, int64_t first,
+                             int64_t count, double *out);
+
+/* Flags of atrip_hip_upload_stream. */
+enum {
+  ATRIP_STREAM_WITH_J = 1,   /* Jijka and Jabci are read ((cT) pass) */
+  ATRIP_STREAM_COMPLEX = 2   /* complex values, as atrip_hip_upload_complex */
+};
+
+/* Upload through a reader, so that the caller never holds a whole tensor in
+ * host memory: each large tensor is requested in slabs of its slowest CTF
+ * index (Tabij and Vabij per j, Vijka and Jijka per a, Vabci and Jabci per
+ * i), through a pinned host buffer of at most min(stage_kb, 256 MiB) bytes,
+ * and re-laid out on the device chunk by chunk; ei, ea and Tai are read
+ * whole.  Replaces the CTF slicing of the reference
+ * (Tensor::slice per SliceUnion, src/atrip/Unions.hpp:22-75) and its
+ * read_all of the small tensors (src/atrip/Atrip.cxx:176-181).  Same
+ * resident system, bit for bit, as atrip_hip_upload(_complex) of the same
+ * values; honours operand_store like the other uploads. */
+int atrip_hip_upload_stream(atrip_hip_ctx *ctx, int64_t No, int64_t Nv,
+                            int flags, atrip_read_fn read, void *user,
+                            int *sys);
+
 /* Complex systems (atrip::Atrip::run<Complex>, src/atrip/Atrip.cxx:1136):
  * the desc pointers address interleaved (re, im) pairs, i.e. the memory of
  * std::complex<double> CTF tensors.  Runs of a complex system use FP64
@@ -120,6 +171,10 @@ int atrip_hip_tuple_energies(atrip_hip_ctx *ctx, int sys, int precision,
  *   "batch"   windows per launch
  *   "timing"  1: time every hot-kernel launch with HIP events
  *   "tile"    doubles-kernel tile variant 0..7, -1 (default) = automatic
+ *   "stage_kb"  device staging buffer of host uploads, KiB (default 4 GiB;
+ *               at least one slab of the slowest CTF index is staged per
+ *               step whatever the budget), also the bound of the pinned
+ *               host buffer of atrip_hip_upload_stream (at most 256 MiB)
  *   "operand_store"  how the next uploads keep the doubles operands:
  *                    0 FP64 (split or bf16x3 copies made by the first
  *                    mixed run of that form), 1 split FP32 only, 2 FP32

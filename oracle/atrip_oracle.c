@@ -475,7 +475,61 @@ typedef struct {
 #define ORACLE_CLONES
 #endif
 
-ORACLE_CLONES static void *worker(void *p) {
+/* Per-tuple arithmetic shared by the whole-tensor and the synthetic-slice
+ * loops: doubles (src/atrip/Atrip.cxx:855-885), Z = copy of W plus singles
+ * (:895-922), the energy selector (:636-670) and, with J slices, the (cT)
+ * pass (:927-963).  work holds 5 No^3 doubles. */
+ORACLE_CLONES static void tuple_compute(size_t No, size_t Nv, size_t a,
+                                        size_t b, size_t c,
+                                        const tuple_slices *s,
+                                        const double *const *VJ,
+                                        const double *const *VhJ,
+                                        const double *Tai, const double *ei,
+                                        const double *ea, int use_gemm,
+                                        int mistake, double *work, double *e_out,
+                                        double *ect_out) {
+  const size_t N3 = No * No * No;
+  double *Tijk = work, *Zijk = work + N3, *tbuf = work + 2 * N3,
+         *vbuf = work + 3 * N3, *TJ = work + 4 * N3;
+  if (use_gemm && mistake == ORACLE_OK)
+    oracle_doubles_gemm(No, Nv, s->V[0], s->V[1], s->V[2], s->V[3], s->V[4],
+                        s->V[5], s->Vh[0], s->Vh[1], s->Vh[2], s->T[0],
+                        s->T[1], s->T[2], s->T2[0], s->T2[1], s->T2[2], Tijk,
+                        tbuf, vbuf);
+  else
+    doubles_loop_impl(No, Nv, s->V[0], s->V[1], s->V[2], s->V[3], s->V[4],
+                      s->V[5], s->Vh[0], s->Vh[1], s->Vh[2], s->T[0], s->T[1],
+                      s->T[2], s->T2[0], s->T2[1], s->T2[2], Tijk, mistake);
+  memcpy(Zijk, Tijk, N3 * sizeof(double));
+  singles_impl(No, Nv, a, b, c, Tai, s->V2[0], s->V2[1], s->V2[2], Zijk,
+               mistake);
+  int distinct = 0;
+  if (a == b) distinct++;
+  if (b == c) distinct--;
+  if (mistake == ORACLE_MISTAKE_SAME_AS_DIST) distinct = 0;
+  const double epsabc = (ea[a] + ea[b]) + ea[c];
+  const double e =
+      distinct == 0 ? energy_distinct_impl(epsabc, No, ei, Tijk, Zijk, mistake)
+                    : energy_same_impl(epsabc, No, ei, Tijk, Zijk, mistake);
+  double ect = e;
+  if (VJ) {
+    if (use_gemm && mistake == ORACLE_OK)
+      oracle_doubles_gemm(No, Nv, VJ[0], VJ[1], VJ[2], VJ[3], VJ[4], VJ[5],
+                          VhJ[0], VhJ[1], VhJ[2], s->T[0], s->T[1], s->T[2],
+                          s->T2[0], s->T2[1], s->T2[2], TJ, tbuf, vbuf);
+    else
+      doubles_loop_impl(No, Nv, VJ[0], VJ[1], VJ[2], VJ[3], VJ[4], VJ[5],
+                        VhJ[0], VhJ[1], VhJ[2], s->T[0], s->T[1], s->T[2],
+                        s->T2[0], s->T2[1], s->T2[2], TJ, mistake);
+    ect = distinct == 0
+              ? energy_distinct_impl(epsabc, No, ei, TJ, Zijk, mistake)
+              : energy_same_impl(epsabc, No, ei, TJ, Zijk, mistake);
+  }
+  *e_out = e;
+  *ect_out = ect;
+}
+
+static void *worker(void *p) {
   worker_args *w = (worker_args *)p;
   const oracle_input *in = w->in;
   const size_t No = in->No, Nv = in->Nv, N3 = No * No * No, NvNo = Nv * No;
@@ -486,8 +540,6 @@ ORACLE_CLONES static void *worker(void *p) {
     w->status = -1;
     return NULL;
   }
-  double *Tijk = buf, *Zijk = buf + N3, *tbuf = buf + 2 * N3,
-         *vbuf = buf + 3 * N3, *TJ = buf + 4 * N3;
   double *sl = buf + 5 * N3, *slJ = sl + oracle_slices_size(No, Nv);
   const int have_J = in->Jijka && in->Jabci;
   for (uint64_t it = w->lo; it < w->hi; it++) {
@@ -511,54 +563,18 @@ ORACLE_CLONES static void *worker(void *p) {
       s.T2[q] = t2;
       s.V2[q] = v2;
     }
-    /* doubles, src/atrip/Atrip.cxx:855-885 */
-    if (w->use_gemm && w->mistake == ORACLE_OK)
-      oracle_doubles_gemm(No, Nv, s.V[0], s.V[1], s.V[2], s.V[3], s.V[4],
-                          s.V[5], s.Vh[0], s.Vh[1], s.Vh[2], s.T[0], s.T[1],
-                          s.T[2], s.T2[0], s.T2[1], s.T2[2], Tijk, tbuf, vbuf);
-    else
-      doubles_loop_impl(No, Nv, s.V[0], s.V[1], s.V[2], s.V[3], s.V[4],
-                        s.V[5], s.Vh[0], s.Vh[1], s.Vh[2], s.T[0], s.T[1],
-                        s.T[2], s.T2[0], s.T2[1], s.T2[2], Tijk, w->mistake);
-    /* Z = copy of W, then singles, src/atrip/Atrip.cxx:895-922 */
-    memcpy(Zijk, Tijk, N3 * sizeof(double));
-    singles_impl(No, Nv, a, b, c, w->Tai, s.V2[0], s.V2[1], s.V2[2], Zijk,
-                 w->mistake);
-    /* energy selector, src/atrip/Atrip.cxx:636-670 */
-    int distinct = 0;
-    if (a == b) distinct++;
-    if (b == c) distinct--;
-    if (w->mistake == ORACLE_MISTAKE_SAME_AS_DIST) distinct = 0;
-    const double epsabc = (in->ea[a] + in->ea[b]) + in->ea[c];
-    const double e =
-        distinct == 0
-            ? energy_distinct_impl(epsabc, No, in->ei, Tijk, Zijk, w->mistake)
-            : energy_same_impl(epsabc, No, in->ei, Tijk, Zijk, w->mistake);
-    double ect = e;
+    const double *VJ[6], *VhJ[3];
     if (have_J) {
-      /* (cT) pass, src/atrip/Atrip.cxx:927-963 */
-      const double *VJ[6];
       for (int q = 0; q < 6; q++) {
         double *dst = slJ + q * NvNo;
         cut_vph(in->Jabci, No, Nv, xy[q][0], xy[q][1], dst);
         VJ[q] = dst;
       }
-      if (w->use_gemm && w->mistake == ORACLE_OK)
-        oracle_doubles_gemm(No, Nv, VJ[0], VJ[1], VJ[2], VJ[3], VJ[4], VJ[5],
-                            in->Jijka + N3 * a, in->Jijka + N3 * b,
-                            in->Jijka + N3 * c, s.T[0], s.T[1], s.T[2],
-                            s.T2[0], s.T2[1], s.T2[2], TJ, tbuf, vbuf);
-      else
-        doubles_loop_impl(No, Nv, VJ[0], VJ[1], VJ[2], VJ[3], VJ[4], VJ[5],
-                          in->Jijka + N3 * a, in->Jijka + N3 * b,
-                          in->Jijka + N3 * c, s.T[0], s.T[1], s.T[2],
-                          s.T2[0], s.T2[1], s.T2[2], TJ, w->mistake);
-      ect = distinct == 0
-                ? energy_distinct_impl(epsabc, No, in->ei, TJ, Zijk, w->mistake)
-                : energy_same_impl(epsabc, No, in->ei, TJ, Zijk, w->mistake);
+      for (int q = 0; q < 3; q++) VhJ[q] = in->Jijka + N3 * abc[q];
     }
-    w->per_tuple[it - w->first] = e;
-    w->per_tuple_ct[it - w->first] = ect;
+    tuple_compute(No, Nv, a, b, c, &s, have_J ? VJ : NULL, VhJ, w->Tai, in->ei,
+                  in->ea, w->use_gemm, w->mistake, buf,
+                  &w->per_tuple[it - w->first], &w->per_tuple_ct[it - w->first]);
   }
   free(buf);
   w->status = 0;
@@ -796,7 +812,8 @@ static void *worker_z(void *p) {
     int distinct = 0;
     if (a == b) distinct++;
     if (b == c) distinct--;
-    const zdouble epsabc = (ea[a] + ea[b]) + ea[c];
+    /* real part only, src/atrip/Atrip.cxx:643-644 (std::real of the sum) */
+    const zdouble epsabc = creal((ea[a] + ea[b]) + ea[c]);
     const double e = energy_z(distinct, epsabc, No, ei, Tijk, Zijk);
     double ect = e;
     if (have_J) {
@@ -908,4 +925,194 @@ void oracle_fill_synthetic(uint64_t seed, int tensor, size_t No, size_t Nv,
     for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
   free(args);
   free(th);
+}
+
+/* ------------------------------------------------------------------ */
+/* The (T) loop on slices drawn from the generator                      */
+/* ------------------------------------------------------------------ */
+
+/* Slices of one index x (T(x) and Vhhh(x), plus Jhhh(x) with J), kept for
+ * the next tuples: consecutive tuples share a and mostly b.  A tuple needs
+ * at most 9 distinct ones, and the least recently used is replaced, so the
+ * slices of the current tuple are never evicted while it is set up. */
+#define SYN_CACHE 9
+typedef struct {
+  long x[SYN_CACHE];
+  int kind[SYN_CACHE];
+  double *data[SYN_CACHE];
+  uint64_t used[SYN_CACHE], clock;
+} syn_cache;
+
+typedef struct {
+  uint64_t seed;
+  size_t No, Nv;
+  int with_j, use_gemm;
+  double scale;
+  const double *Tai, *ei, *ea;
+  uint64_t lo, hi, first;
+  double *per_tuple, *per_tuple_ct;
+  int status;
+} syn_args;
+
+/* kind 0: T(x)[E + Nv i + Nv No j] = Tabij[x, E, i, j] (Unions.hpp:78-113);
+ * kind 1: Vhhh(x)[q] = Vijka[q + No^3 x]; kind 2: the same of Jijka. */
+static const double *syn_x_slice(const syn_args *w, syn_cache *c, int kind,
+                                 size_t x) {
+  const size_t No = w->No, Nv = w->Nv, N3 = No * No * No;
+  int slot = 0;
+  for (int q = 0; q < SYN_CACHE; q++) {
+    if (c->data[q] && c->kind[q] == kind && c->x[q] == (long)x) {
+      c->used[q] = ++c->clock;
+      return c->data[q];
+    }
+    if (c->used[q] < c->used[slot]) slot = q;
+  }
+  const size_t n = kind == 0 ? Nv * No * No : N3;
+  free(c->data[slot]);
+  double *d = (double *)malloc(sizeof(double) * n);
+  c->data[slot] = d;
+  c->kind[slot] = kind;
+  c->x[slot] = (long)x;
+  c->used[slot] = ++c->clock;
+  if (!d) return NULL;
+  if (kind == 0) {
+    const uint64_t st = atrip_synth_stream(w->seed, ATRIP_SYN_TABIJ);
+    for (size_t r = 0; r < n; r++)  /* r = E + Nv (i + No j) */
+      d[r] = atrip_synth_value(st, ATRIP_SYN_TABIJ, No, Nv, x + Nv * r) *
+             w->scale;
+  } else {
+    const int t = kind == 1 ? ATRIP_SYN_VIJKA : ATRIP_SYN_JIJKA;
+    const uint64_t st = atrip_synth_stream(w->seed, t);
+    for (size_t q = 0; q < N3; q++)
+      d[q] = atrip_synth_value(st, t, No, Nv, q + N3 * x) * w->scale;
+  }
+  return d;
+}
+
+static void *syn_worker(void *p) {
+  syn_args *w = (syn_args *)p;
+  const size_t No = w->No, Nv = w->Nv, N3 = No * No * No, NvNo = Nv * No;
+  const size_t NoNo = No * No, NvNv = Nv * Nv;
+  syn_cache cache;
+  memset(&cache, 0, sizeof cache);
+  double *buf = (double *)malloc(
+      sizeof(double) * (5 * N3 + 2 * oracle_slices_size(No, Nv)));
+  w->status = -1;
+  if (!buf) return NULL;
+  double *sl = buf + 5 * N3, *slJ = sl + oracle_slices_size(No, Nv);
+  const uint64_t sV = atrip_synth_stream(w->seed, ATRIP_SYN_VABCI);
+  const uint64_t sJ = atrip_synth_stream(w->seed, ATRIP_SYN_JABCI);
+  const uint64_t sT = atrip_synth_stream(w->seed, ATRIP_SYN_TABIJ);
+  const uint64_t sW = atrip_synth_stream(w->seed, ATRIP_SYN_VABIJ);
+  int ok = 1;
+  for (uint64_t it = w->lo; it < w->hi && ok; it++) {
+    uint64_t abc[3];
+    oracle_nth_tuple(it, Nv, abc);
+    const size_t a = abc[0], b = abc[1], c = abc[2];
+    const size_t xy[6][2] = {{a, b}, {a, c}, {b, c}, {b, a}, {c, a}, {c, b}};
+    tuple_slices s;
+    const double *VJ[6], *VhJ[3];
+    /* V(xy)ph[E + Nv i] = Vabci[x + Nv y + Nv^2 E + Nv^3 i] (cut_vph) */
+    for (int q = 0; q < 6; q++) {
+      double *dst = sl + q * NvNo, *dJ = slJ + q * NvNo;
+      for (size_t i = 0; i < No; i++)
+        for (size_t E = 0; E < Nv; E++) {
+          const uint64_t idx = xy[q][0] + Nv * xy[q][1] + NvNv * E + NvNv * Nv * i;
+          dst[E + Nv * i] =
+              atrip_synth_value(sV, ATRIP_SYN_VABCI, No, Nv, idx) * w->scale;
+          if (w->with_j)
+            dJ[E + Nv * i] =
+                atrip_synth_value(sJ, ATRIP_SYN_JABCI, No, Nv, idx) * w->scale;
+        }
+      s.V[q] = dst;
+      VJ[q] = dJ;
+    }
+    /* X(xy)hh[i + No j] = Xpphh[x + Nv y + Nv^2 i + Nv^2 No j] (cut_hh) */
+    for (int q = 0; q < 3; q++) {
+      double *t2 = sl + 6 * NvNo + q * NoNo;
+      double *v2 = sl + 6 * NvNo + 3 * NoNo + q * NoNo;
+      for (size_t j = 0; j < No; j++)
+        for (size_t i = 0; i < No; i++) {
+          const uint64_t idx = xy[q][0] + Nv * xy[q][1] + NvNv * i + NvNv * No * j;
+          t2[i + No * j] =
+              atrip_synth_value(sT, ATRIP_SYN_TABIJ, No, Nv, idx) * w->scale;
+          v2[i + No * j] =
+              atrip_synth_value(sW, ATRIP_SYN_VABIJ, No, Nv, idx) * w->scale;
+        }
+      s.T2[q] = t2;
+      s.V2[q] = v2;
+    }
+    for (int q = 0; q < 3; q++) {
+      s.T[q] = syn_x_slice(w, &cache, 0, abc[q]);
+      s.Vh[q] = syn_x_slice(w, &cache, 1, abc[q]);
+      VhJ[q] = w->with_j ? syn_x_slice(w, &cache, 2, abc[q]) : NULL;
+      if (!s.T[q] || !s.Vh[q] || (w->with_j && !VhJ[q])) ok = 0;
+    }
+    if (ok)
+      tuple_compute(No, Nv, a, b, c, &s, w->with_j ? VJ : NULL, VhJ, w->Tai,
+                    w->ei, w->ea, w->use_gemm, ORACLE_OK, buf,
+                    &w->per_tuple[it - w->first],
+                    &w->per_tuple_ct[it - w->first]);
+  }
+  for (int q = 0; q < SYN_CACHE; q++) free(cache.data[q]);
+  free(buf);
+  w->status = ok ? 0 : -1;
+  return NULL;
+}
+
+int oracle_run_synthetic(uint64_t seed, size_t No, size_t Nv, int with_j,
+                         double scale, int ijkabc, uint64_t first,
+                         uint64_t count, int use_gemm, int nthreads,
+                         double *per_tuple, double *per_tuple_ct) {
+  const uint64_t n = oracle_n_tuples(Nv);
+  if (first > n) first = n;
+  if (count > n - first) count = n - first;
+  if (nthreads < 1) nthreads = 1;
+  double *Tai = (double *)malloc(sizeof(double) * Nv * No);
+  double *ei = (double *)malloc(sizeof(double) * No);
+  double *ea = (double *)malloc(sizeof(double) * Nv);
+  syn_args *args = (syn_args *)calloc(nthreads, sizeof(syn_args));
+  pthread_t *th = (pthread_t *)calloc(nthreads, sizeof(pthread_t));
+  if (!Tai || !ei || !ea || !args || !th) return -1;
+  const uint64_t sTai = atrip_synth_stream(seed, ATRIP_SYN_TAI);
+  for (size_t q = 0; q < Nv * No; q++) {
+    const double v = atrip_synth_value(sTai, ATRIP_SYN_TAI, No, Nv, q) * scale;
+    Tai[q] = ijkabc ? -v : v;  /* src/atrip/Atrip.cxx:183-187 */
+  }
+  oracle_fill_synthetic(seed, ATRIP_SYN_EI, No, Nv, ei, No, 1);
+  oracle_fill_synthetic(seed, ATRIP_SYN_EA, No, Nv, ea, Nv, 1);
+  const uint64_t chunk = (count + nthreads - 1) / nthreads;
+  for (int t = 0; t < nthreads; t++) {
+    syn_args *w = &args[t];
+    w->seed = seed;
+    w->No = No;
+    w->Nv = Nv;
+    w->with_j = with_j;
+    w->use_gemm = use_gemm;
+    w->scale = scale;
+    w->Tai = Tai;
+    w->ei = ei;
+    w->ea = ea;
+    w->first = first;
+    w->lo = first + chunk * t;
+    w->hi = w->lo + chunk;
+    if (w->lo > first + count) w->lo = first + count;
+    if (w->hi > first + count) w->hi = first + count;
+    w->per_tuple = per_tuple;
+    w->per_tuple_ct = per_tuple_ct;
+    if (nthreads == 1)
+      syn_worker(w);
+    else
+      pthread_create(&th[t], NULL, syn_worker, w);
+  }
+  int status = 0;
+  if (nthreads > 1)
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  for (int t = 0; t < nthreads; t++) status |= args[t].status;
+  free(args);
+  free(th);
+  free(Tai);
+  free(ei);
+  free(ea);
+  return status;
 }

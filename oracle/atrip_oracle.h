@@ -152,6 +152,21 @@ int oracle_run_z(const oracle_input_z *in, int ijkabc, uint64_t first,
                  uint64_t count, int nthreads, double *energy,
                  double *ct_energy, double *per_tuple, double *per_tuple_ct);
 
+/*
+ * The same loop as oracle_run (DGEMM or loop form, same per-tuple
+ * arithmetic), on the seeded synthetic system of include/atrip_synth.h with
+ * every amplitude and integral times `scale`, but building each tuple's
+ * slices straight from the generator instead of cutting them out of whole
+ * tensors: for shapes whose tensors do not fit host memory (No=60, Nv=600
+ * holds 125 GB).  Writes the raw per-tuple energies (before the sign) of
+ * tuples [first, first+count) to per_tuple / per_tuple_ct.  Pinned by
+ * equality with oracle_run on the whole synthetic tensors (tests).
+ */
+int oracle_run_synthetic(uint64_t seed, size_t No, size_t Nv, int with_j,
+                         double scale, int ijkabc, uint64_t first,
+                         uint64_t count, int use_gemm, int nthreads,
+                         double *per_tuple, double *per_tuple_ct);
+
 /* Fill a whole tensor in CTF layout from the counter-based generator
  * (include/atrip_synth.h).  n = number of elements of that tensor; the
  * elements are split over nthreads threads. */

@@ -69,6 +69,10 @@ def lib():
         L.oracle_energy_distinct.argtypes = [C.c_double, C.c_size_t, dptr, dptr, dptr]
         L.oracle_energy_same.restype = C.c_double
         L.oracle_energy_same.argtypes = [C.c_double, C.c_size_t, dptr, dptr, dptr]
+        L.oracle_run_synthetic.restype = C.c_int
+        L.oracle_run_synthetic.argtypes = [
+            C.c_uint64, C.c_size_t, C.c_size_t, C.c_int, C.c_double, C.c_int,
+            C.c_uint64, C.c_uint64, C.c_int, C.c_int, dptr, dptr]
         L.oracle_run_z.restype = C.c_int
         L.oracle_run_z.argtypes = [
             C.POINTER(OracleInput), C.c_int, C.c_uint64, C.c_uint64, C.c_int,
@@ -194,6 +198,24 @@ def run(no: int, nv: int, t: dict, *, ijkabc=False, first=0, count=None,
     if per_tuple:
         return e.value, ect.value, pt[:count], ptc[:count]
     return e.value, ect.value
+
+
+def run_synthetic(no: int, nv: int, seed: int, first: int, count: int, *,
+                  with_j=False, scale=1.0, ijkabc=False, use_gemm=True, nthreads=1):
+    """Raw per-tuple energies (and (cT) energies) of tuples [first,
+    first+count) of the seeded synthetic system, slices drawn from the
+    generator per tuple (no whole tensors in host memory)."""
+    n = n_tuples(nv)
+    first = min(first, n)
+    count = max(0, min(count, n - first))
+    pt = np.zeros(max(count, 1))
+    ptc = np.zeros(max(count, 1))
+    rc = lib().oracle_run_synthetic(seed, no, nv, int(with_j), float(scale),
+                                    int(ijkabc), first, count, int(use_gemm),
+                                    nthreads, _p(pt), _p(ptc))
+    if rc != 0:
+        raise RuntimeError(f"oracle_run_synthetic failed ({rc})")
+    return pt[:count], ptc[:count]
 
 
 def blas_init(path=None):

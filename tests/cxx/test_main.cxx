@@ -181,6 +181,34 @@ int main(int argc, char **argv) {
            assert(close(out.energy, ref.first));
            assert(out.ct_energy == out.energy);)
 
+  TESTCASE("large tensors are read in slabs through a bounded buffer",
+           // 4 KiB of staging: 256 doubles per pinned half, so every large
+           // tensor arrives in many ranged reads of at most 256 values and
+           // the energies are those of the whole-tensor upload
+           System s(6, 13, 21, true);
+           const uint64_t n = oracle_n_tuples(13);
+           auto ref = s.oracle(0, n);
+           auto out = atrip::Atrip::run(s.input().with_stage_kb(4));
+           assert(close(out.energy, ref.first));
+           assert(close(out.ct_energy, ref.second));
+           for (int k = 3; k < 9; k++) {
+             assert(s.t[k]->range_reads > 1);
+             assert(s.t[k]->max_range_read <= 256);
+           }
+           auto whole = atrip::Atrip::run(s.input());
+           assert(whole.energy == out.energy && whole.ct_energy == out.ct_energy);)
+
+  TESTCASE("complex tensors are read in slabs too",
+           ComplexSystem s(4, 9, 29, true);
+           auto ref = s.oracle();
+           auto out = atrip::Atrip::run(s.input().with_stage_kb(4));
+           assert(close(out.energy, ref.first));
+           assert(close(out.ct_energy, ref.second));
+           for (int k = 3; k < 9; k++) {
+             assert(s.t[k]->range_reads > 1);
+             assert(s.t[k]->max_range_read <= 128);
+           })
+
   TESTCASE("max_iterations stops after tuple max_iterations",
            System s(5, 11, 22, false);
            auto ref = s.oracle(0, 101);  // tuples 0..100

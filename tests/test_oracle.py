@@ -199,10 +199,12 @@ def test_cT_with_J_equal_V(oracle):
 
 # ---- complex (T) ---------------------------------------------------------
 
-def _full_space_energy_z(no, nv, t, conj_holes=True, conj_w=True):
+def _full_space_energy_z(no, nv, t, conj_holes=True, conj_w=True, real_ea=True):
     """Complex analogue of _full_space_energy: hole terms with conj(Vhhh),
-    energy -(1/6) Re sum 2 conj(W) Y / D.  The switches drop either
-    conjugation (used to show the identity tells them apart)."""
+    energy -(1/6) Re sum 2 conj(W) Y / D, with D = Re(ea_a + ea_b + ea_c)
+    - (ei_i + ei_j + ei_k) (the reference takes the real part of the virtual
+    sum only).  The switches drop either conjugation or keep Im(ea) (used to
+    show the identity tells them apart)."""
     T = t["Tabij"].reshape((nv, nv, no, no), order="F")
     Vab = t["Vabij"].reshape((nv, nv, no, no), order="F")
     Vh = t["Vijka"].reshape((no, no, no, nv), order="F")
@@ -223,8 +225,9 @@ def _full_space_energy_z(no, nv, t, conj_holes=True, conj_w=True):
     tr = lambda X, p: X.transpose(0, 1, 2, *[3 + q for q in p])
     Y = (4 * Z + tr(Z, (1, 2, 0)) + tr(Z, (2, 0, 1)) - 2 * tr(Z, (0, 2, 1))
          - 2 * tr(Z, (1, 0, 2)) - 2 * tr(Z, (2, 1, 0)))
-    D = (ea[:, None, None, None, None, None] + ea[None, :, None, None, None, None]
-         + ea[None, None, :, None, None, None] - ei[None, None, None, :, None, None]
+    ev = np.real(ea) if real_ea else ea
+    D = (ev[:, None, None, None, None, None] + ev[None, :, None, None, None, None]
+         + ev[None, None, :, None, None, None] - ei[None, None, None, :, None, None]
          - ei[None, None, None, None, :, None] - ei[None, None, None, None, None, :])
     Wc = np.conj(W) if conj_w else W
     return -np.real(np.sum(2 * Wc * Y / D)) / 6.0
@@ -245,6 +248,20 @@ def test_complex_identity_sees_each_conjugation(oracle):
     e, _ = oracle.run_z(no, nv, t)
     for kw in ({"conj_holes": False}, {"conj_w": False}):
         assert abs(e - _full_space_energy_z(no, nv, t, **kw)) > 1e-5 * abs(e), kw
+
+
+def test_complex_epsabc_is_real_part(oracle):
+    """Orbital energies with imaginary parts: the identity holds with the
+    real part of ea_a + ea_b + ea_c and complex ei in the denominator
+    (src/atrip/Atrip.cxx:643-644), and keeping Im(ea) would fail it."""
+    no, nv = 4, 7
+    t = oracle.synthetic_z(no, nv, seed=13)
+    rng = np.random.default_rng(5)
+    t["ea"] = t["ea"] + 1j * rng.uniform(0.05, 0.3, nv)
+    t["ei"] = t["ei"] + 1j * rng.uniform(-0.2, 0.2, no)
+    e, _ = oracle.run_z(no, nv, t)
+    assert abs(e - _full_space_energy_z(no, nv, t)) <= 1e-12 * abs(e)
+    assert abs(e - _full_space_energy_z(no, nv, t, real_ea=False)) > 1e-6 * abs(e)
 
 
 def test_complex_with_real_inputs_is_the_real_path(oracle):
@@ -281,3 +298,33 @@ def test_complex_cT_and_ijkabc(oracle):
     tn["Tai"] = -t["Tai"]
     e2, _ = oracle.run_z(no, nv, tn)
     assert abs(e1 + e2) <= 1e-14 * abs(e1)  # ijkabc: Tai negated, no final sign
+
+
+# ---- the slice-wise synthetic loop (large shapes) -------------------------
+
+@pytest.mark.parametrize("no,nv,seed,with_j,scale", [
+    (5, 11, 3, False, 1.0), (6, 13, 4, True, 1.0), (7, 9, 5, True, 0.37),
+    (4, 20, 6, False, 1.0)])
+def test_slice_wise_synthetic_loop_is_the_whole_tensor_loop(oracle, no, nv, seed,
+                                                            with_j, scale):
+    """oracle_run_synthetic draws each tuple's slices from the generator;
+    on whole synthetic tensors (amplitudes and integrals times `scale`) the
+    whole-tensor loop gives the same per-tuple bits, in both forms and on a
+    range that starts mid-space."""
+    t = oracle.synthetic(no, nv, seed, with_j=with_j)
+    for k in ("Tai", "Tabij", "Vabij", "Vijka", "Vabci", "Jijka", "Jabci"):
+        if k in t:
+            t[k] = t[k] * scale
+    n = oracle.n_tuples(nv)
+    for gemm in (False, True):
+        _, _, pt, ptc = oracle.run(no, nv, t, per_tuple=True, use_gemm=gemm)
+        for first, count in [(0, n), (n // 3, 17)]:
+            a, b = oracle.run_synthetic(no, nv, seed, first, count, with_j=with_j,
+                                        scale=scale, use_gemm=gemm, nthreads=3)
+            assert np.array_equal(a, pt[first:first + count])
+            assert np.array_equal(b, ptc[first:first + count])
+    # ijkabc negates Tai in both
+    _, _, pt, _ = oracle.run(no, nv, t, per_tuple=True, ijkabc=True)
+    a, _ = oracle.run_synthetic(no, nv, seed, 0, n, with_j=with_j, scale=scale,
+                                ijkabc=True, use_gemm=False)
+    assert np.array_equal(a, pt)
