@@ -632,6 +632,7 @@ void ensure(void *&ptr, size_t &have, size_t need, const char *what) {
 
 struct RunResult {
   double e = 0, ect = 0;
+  uint64_t tuples = 0;  // tuples computed
 };
 
 // Destroys the queued timing events (read at the end of a run, or left by
@@ -653,9 +654,17 @@ void drop_pending_events(atrip_hip_ctx *ctx) {
 
 // Runs every batch of windows intersecting [first, last).  When `sink` is
 // given it receives the per-slot energies of each batch.
+// A fixed device region a run may carve its scratch from (no allocation).
+struct FixedScratch {
+  void *base;
+  size_t bytes;
+};
+
 template <typename Sink>
 RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
-                    uint64_t first, uint64_t last, Sink &&sink) {
+                    uint64_t first, uint64_t last, Sink &&sink,
+                    uint64_t wstride = 1, uint64_t woffset = 0,
+                    const FixedScratch *fixed = nullptr) {
   drop_pending_events(ctx);
   DevSystem &s = get_sys(ctx, sys);
   WindowPlan &plan = get_plan(ctx, sys, s);
@@ -676,9 +685,16 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   std::vector<int32_t> sel;
   std::vector<uint64_t> counts;
   const size_t nwin = plan.windows.size() / 3;
+  int64_t pair = -1, prev_al = -1, prev_be = -1;  // (alpha, beta) row index
   for (size_t w = 0; w < nwin; w++) {
     const int64_t al = plan.windows[3 * w], be = plan.windows[3 * w + 1],
                   ga = plan.windows[3 * w + 2];
+    if (al != prev_al || be != prev_be) {
+      pair++;
+      prev_al = al;
+      prev_be = be;
+    }
+    if (wstride > 1 && (uint64_t)pair % wstride != woffset % wstride) continue;
     uint64_t cnt = 0;
     if (full || (plan.wmin[w] >= first && plan.wmax[w] < last)) {
       cnt = plan.wcount[w];
@@ -698,50 +714,105 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
     if (!cnt) continue;
     sel.insert(sel.end(), {(int32_t)al, (int32_t)be, (int32_t)ga});
     counts.push_back(cnt);
+    res.tuples += cnt;
   }
   const int nsel = (int)counts.size();
   if (!nsel) return res;
-  void *d_sel = nullptr;
-  check(hipMalloc(&d_sel, sel.size() * sizeof(int32_t)), "alloc windows");
-  struct FreeOnExit {
-    void *p;
-    ~FreeOnExit() { (void)hipFree(p); }
-  } guard{d_sel};
-  check(hipMemcpy(d_sel, sel.data(), sel.size() * sizeof(int32_t),
-                  hipMemcpyHostToDevice),
-        "upload windows");
-
-  const int bw = std::max(1, ctx->batch);
+  // Device scratch of the run: class buffers G (and those of the (cT)
+  // pass; four sets for complex systems), per-slot energies, W cubes, batch
+  // partials and the window list.  Taken from the context's buffers (grown
+  // on demand), or carved out of a fixed region without any allocation
+  // (then the windows per launch shrink until the run fits).
   const int64_t nb3 = (int64_t)nb * nb * nb;
-  const int64_t slots_max = (int64_t)std::min(bw, nsel) * nb3;
   const int64_t W3 = s.No * s.No * s.No;
   const bool use_j = s.has_j;
-  const size_t gsz =
-      (size_t)(3 * slots_max * W3) * (g32 ? sizeof(float) : sizeof(double));
-  // complex: four class-buffer sets per pass (Wr = G0 + G1, Wi = G2 + G3)
   const int nsets = s.cplx ? 4 : 1;
-  ensure((void *&)ctx->G, ctx->G_bytes, (use_j ? 2 : 1) * nsets * gsz,
-         "alloc G");
+  const size_t gelem = g32 ? sizeof(float) : sizeof(double);
+  struct Sizes {
+    int64_t slots_max = 0;
+    size_t gsz = 0, g = 0, e_slot = 0, wcube = 0, partials = 0, windows = 0;
+    int nbatch = 0;
+  };
+  auto sizes = [&](int bw_) {
+    Sizes z;
+    z.slots_max = (int64_t)std::min(bw_, nsel) * nb3;
+    z.gsz = (size_t)(3 * z.slots_max * W3) * gelem;
+    z.g = (size_t)((use_j ? 2 : 1) * nsets) * z.gsz;
+    z.e_slot = (size_t)(2 * z.slots_max) * sizeof(double);
+    z.wcube = (size_t)((use_j ? 2 : 1) * (s.cplx ? 2 : 1) * z.slots_max *
+                       w_cube(s.No)) *
+              sizeof(double);
+    z.nbatch = (nsel + bw_ - 1) / bw_;
+    z.partials = (size_t)(4 * z.nbatch) * sizeof(double);
+    z.windows = sel.size() * sizeof(int32_t);
+    return z;
+  };
+  auto a256 = [](size_t x) { return (x + 255) / 256 * 256; };
+  auto total = [&](const Sizes &z) {
+    return a256(z.g) + a256(z.e_slot) + a256(z.wcube) + a256(z.partials) +
+           a256(z.windows);
+  };
+  int bw = std::max(1, ctx->batch);
+  void *Gbuf = nullptr, *e_slot = nullptr, *wcube = nullptr,
+       *partials = nullptr, *d_sel = nullptr;
+  struct FreeOnExit {
+    void *p = nullptr;
+    ~FreeOnExit() {
+      if (p) (void)hipFree(p);
+    }
+  } guard;
+  Sizes z;
+  if (fixed) {
+    while (bw > 1 && total(sizes(bw)) > fixed->bytes) bw--;
+    z = sizes(bw);
+    if (total(z) > fixed->bytes)
+      throw Fail{"run needs " + std::to_string(total(z)) +
+                 " bytes of scratch for one window of edge " +
+                 std::to_string(nb) + " at No=" + std::to_string(s.No) +
+                 "; the allocation-free entry point holds " +
+                 std::to_string(fixed->bytes) + " (use a smaller window)"};
+    char *p = (char *)fixed->base;
+    Gbuf = p;
+    p += a256(z.g);
+    e_slot = p;
+    p += a256(z.e_slot);
+    wcube = p;
+    p += a256(z.wcube);
+    partials = p;
+    p += a256(z.partials);
+    d_sel = p;
+    check(hipMemcpyAsync(d_sel, sel.data(), z.windows, hipMemcpyHostToDevice,
+                         ctx->stream),
+          "upload windows");
+  } else {
+    z = sizes(bw);
+    check(hipMalloc(&d_sel, z.windows), "alloc windows");
+    guard.p = d_sel;
+    check(hipMemcpy(d_sel, sel.data(), z.windows, hipMemcpyHostToDevice),
+          "upload windows");
+    ensure((void *&)ctx->G, ctx->G_bytes, z.g, "alloc G");
+    ensure((void *&)ctx->e_slot, ctx->e_slot_bytes, z.e_slot, "alloc e_slot");
+    ensure((void *&)ctx->wcube, ctx->wcube_bytes, z.wcube, "alloc W cubes");
+    ensure((void *&)ctx->partials, ctx->partials_bytes, z.partials,
+           "alloc partials");
+    Gbuf = ctx->G;
+    e_slot = ctx->e_slot;
+    wcube = ctx->wcube;
+    partials = ctx->partials;
+  }
+  const size_t gsz = z.gsz;
+  const int nbatch = z.nbatch;
   DevSystem zpass[4];
   if (s.cplx) complex_passes(s, zpass);
   double *Gz[4], *GJz[4];
   for (int q = 0; q < 4; q++) {
-    Gz[q] = (double *)((char *)ctx->G + q * gsz);
-    GJz[q] = (double *)((char *)ctx->G + (4 + q) * gsz);
+    Gz[q] = (double *)((char *)Gbuf + q * gsz);
+    GJz[q] = (double *)((char *)Gbuf + (4 + q) * gsz);
   }
-  ensure((void *&)ctx->e_slot, ctx->e_slot_bytes,
-         (size_t)(2 * slots_max) * sizeof(double), "alloc e_slot");
-  // W as cubes for the energy kernels: W (and W of the (cT) pass), real and
-  // imaginary parts for complex systems
-  ensure((void *&)ctx->wcube, ctx->wcube_bytes,
-         (size_t)((use_j ? 2 : 1) * (s.cplx ? 2 : 1) * slots_max * w_cube(s.No)) *
-             sizeof(double),
-         "alloc W cubes");
-  const int nbatch = (nsel + bw - 1) / bw;
-  ensure((void *&)ctx->partials, ctx->partials_bytes,
-         (size_t)(4 * nbatch) * sizeof(double), "alloc partials");
-  void *G = ctx->G;
-  void *GJ = use_j ? (void *)((char *)ctx->G + gsz) : nullptr;
+  // timing events are objects the runtime creates: none without allocation
+  const bool timing = ctx->timing && !fixed;
+  void *G = Gbuf;
+  void *GJ = use_j ? (void *)((char *)Gbuf + gsz) : nullptr;
   // complex products are four real ones (the reference bench counts 4x)
   const double flop_tuple = 12.0 * (double)s.No * s.No * s.No * (s.No + s.Nv) *
                             (s.cplx ? 4.0 : 1.0);
@@ -774,7 +845,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
       }
     } ev;
     hipEvent_t &e0 = ev.e[0], &e1 = ev.e[1], &e2 = ev.e[2];
-    if (ctx->timing) {
+    if (timing) {
       check(hipEventCreate(&e0), "event");
       check(hipEventCreate(&e1), "event");
       check(hipEventCreate(&e2), "event");
@@ -802,17 +873,17 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
                                    ctx->stream),
               "doubles (FP32 products) kernel");
     }
-    if (ctx->timing) check(hipEventRecord(e1, ctx->stream), "event");
+    if (timing) check(hipEventRecord(e1, ctx->stream), "event");
     if (s.cplx)
       check(launch_energy_z(s, b, Gz, use_j ? GJz : nullptr,
-                            ijkabc ? -1.0 : 1.0, ctx->e_slot, ctx->wcube,
+                            ijkabc ? -1.0 : 1.0, (double *)e_slot, (double *)wcube,
                             ctx->stream),
             "energy kernel (complex)");
     else
-      check(launch_energy(s, b, G, GJ, g32, ijkabc ? -1.0 : 1.0, ctx->e_slot,
-                          ctx->wcube, ctx->stream),
+      check(launch_energy(s, b, G, GJ, g32, ijkabc ? -1.0 : 1.0, (double *)e_slot,
+                          (double *)wcube, ctx->stream),
             "energy kernel");
-    if (ctx->timing) {
+    if (timing) {
       check(hipEventRecord(e2, ctx->stream), "event");
       ctx->pending_events.push_back({e0, e1});
       ctx->pending_names.push_back(dname);
@@ -822,12 +893,12 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
       ctx->pending_flop.push_back(0.0);
       ev.handed = true;
     }
-    check(launch_batch_reduce(s, b, ctx->e_slot, ctx->partials, bi,
+    check(launch_batch_reduce(s, b, (double *)e_slot, (double *)partials, bi,
                               ctx->stream),
           "reduce kernel");
     if (sink.active()) {
       host_slots.resize(2 * b.slots);
-      check(hipMemcpyAsync(host_slots.data(), ctx->e_slot,
+      check(hipMemcpyAsync(host_slots.data(), (double *)e_slot,
                            host_slots.size() * sizeof(double),
                            hipMemcpyDeviceToHost, ctx->stream),
             "download slots");
@@ -836,7 +907,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
     }
   }
   std::vector<double> part(4 * nbatch);
-  check(hipMemcpyAsync(part.data(), ctx->partials, part.size() * sizeof(double),
+  check(hipMemcpyAsync(part.data(), (double *)partials, part.size() * sizeof(double),
                        hipMemcpyDeviceToHost, ctx->stream),
         "download partials");
   check(hipStreamSynchronize(ctx->stream), "run");
@@ -1090,6 +1161,22 @@ int atrip_hip_run(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
     // src/atrip/Atrip.cxx:1108-1111: E = -sum unless ijkabc
     *energy = ijkabc ? r.e : -r.e;
     if (ct_energy) *ct_energy = ijkabc ? r.ect : -r.ect;
+  });
+}
+
+int atrip_hip_run_windows(atrip_hip_ctx *ctx, int sys, int precision,
+                          int ijkabc, uint64_t stride, uint64_t offset,
+                          double *energy, double *ct_energy,
+                          uint64_t *tuples) {
+  return guarded(ctx, [&] {
+    if (!energy) throw Fail{"null energy pointer"};
+    if (stride == 0) throw Fail{"stride must be positive"};
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    RunResult r = run_range(ctx, sys, precision, ijkabc, 0, ~0ULL, NoSink{},
+                            stride, offset);
+    *energy = ijkabc ? r.e : -r.e;
+    if (ct_energy) *ct_energy = ijkabc ? r.ect : -r.ect;
+    if (tuples) *tuples = r.tuples;
   });
 }
 

@@ -28,6 +28,7 @@ SYMBOLS = (
     "atrip_hip_upload_synthetic_scaled", "atrip_hip_upload_stream",
     "atrip_hip_upload_complex", "atrip_hip_upload_synthetic_complex",
     "atrip_hip_run", "atrip_hip_tuple_energies", "atrip_hip_set_option",
+    "atrip_hip_run_windows",
     "atrip_hip_get_option", "atrip_hip_kernel_stats", "atrip_hip_reset_stats",
     "atrip_hip_sys_info", "atrip_hip_device_count",
     "atrip_hip_batch_run", "atrip_hip_batch_tune", "atrip_hip_batch_plan",
@@ -92,6 +93,9 @@ def load():
     L.atrip_hip_run.argtypes = [
         vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
         C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.atrip_hip_run_windows.argtypes = [
+        vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
+        C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
     L.atrip_hip_tuple_energies.argtypes = [
         vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
         C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -299,6 +303,15 @@ class Context:
             self._h, sys_, PRECISIONS[precision], int(ijkabc), first, cnt,
             C.byref(e), C.byref(ect)), "run")
         return e.value, ect.value
+
+    def run_windows(self, sys_: int, stride: int, offset: int = 0, precision="fp64",
+                    ijkabc=False):
+        """E(T), E(cT) and tuple count of every stride-th window of the plan."""
+        e, ect, nt = C.c_double(), C.c_double(), C.c_uint64()
+        self._check(self._L.atrip_hip_run_windows(
+            self._h, sys_, PRECISIONS[precision], int(ijkabc), stride, offset,
+            C.byref(e), C.byref(ect), C.byref(nt)), "run_windows")
+        return e.value, ect.value, nt.value
 
     def tuple_energies(self, sys_: int, first: int, count: int, precision="fp64",
                        ijkabc=False, with_ct=False):

@@ -7,7 +7,8 @@ tensors generated on the device from a seed and resident in HBM before the
 timed region starts.  A step is one slice of the tuple space: the tuples whose
 first index falls in a contiguous range of index blocks, with the slices
 balanced so that 16 steps cover the whole space once.  The default run
-(16 timed steps) therefore computes the full E(T) of the system.
+(16 timed steps) therefore computes the full E(T) of the system; with more
+steps the E(T) reported is that of the first pass.
 
 Prints one JSON line with tuples/s, TFLOP/s (doubles FLOP definition of the
 reference, 12 No^3 (No+Nv) per tuple), the FP64-MFMA roofline of the doubles
@@ -34,6 +35,16 @@ PEAK_TFLOPS = {"fp64": 78.6, "split_fp32": 157.3, "fp32": 157.3,
 MFMA_PER_FLOP = {"fp64": 1.0, "split_fp32": 1.0, "fp32": 1.0, "bf16x3": 6.0}
 KERNEL_NAME = {"fp64": "doubles", "split_fp32": "doubles_mixed",
                "fp32": "doubles_fp32", "bf16x3": "doubles_bf16x3"}
+
+
+def neumaier(values):
+    """Compensated sum, in order."""
+    s = c = 0.0
+    for x in values:
+        t = s + x
+        c += (s - t) + x if abs(s) >= abs(x) else (x - t) + s
+        s = t
+    return s + c
 
 
 def log(msg: str):
@@ -91,38 +102,93 @@ def measured_traffic(kernel: str, no: int, nv: int):
     return None
 
 
-def cpu_baseline(no: int, nv: int, seed: int, budget_s: float, cores: int):
-    """Oracle (C port of the reference np=1 CPU path, host BLAS dgemm) on the
-    first tuples of the same workload, all host cores as tuple shards."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _timed_oracle(O, no, nv, t, first, count, threads):
+    t0 = time.time()
+    e, _ = O.run(no, nv, t, first=first, count=count, use_gemm=True, nthreads=threads)
+    return time.time() - t0, e
+
+
+def _sized_sample(O, no, nv, t, threads, budget_s):
+    """First K tuples (a-major order, as the reference's max_iterations),
+    K sized from a short probe so that the tuple loop takes about budget_s.
+    The oracle's one-time setup per run (cutting the T(x) slices of the
+    whole Tabij) is timed with an empty range and taken out of the rate."""
+    setup, _ = _timed_oracle(O, no, nv, t, 0, 0, threads)
+    probe = max(2 * threads, 4)
+    dt, _ = _timed_oracle(O, no, nv, t, 0, probe, threads)
+    per = max(dt - setup, 1e-4) / probe
+    count = int(max(probe, min(n_tuples(nv), budget_s / per)))
+    count = max(threads, count // threads * threads)
+    dt, _ = _timed_oracle(O, no, nv, t, 0, count, threads)
+    return count, max(dt - setup, 1e-6), setup
+
+
+def cpu_baseline(no: int, nv: int, seed: int, budget_s: float):
+    """The oracle (C port of the reference np=1 CPU path: per tuple 12 host
+    BLAS dgemm, permuted accumulates, singles, energy) on the host cores: C1
+    (No=10, Nv=50) to completion, and the first tuples of the GPU workload
+    sized to about budget_s, each on 1 core and on the process's CPU share
+    (tuple shards, single-threaded BLAS per thread)."""
     from oracle import oracle as O  # the checker / CPU baseline only
 
     blas = O.blas_init()
+    affinity = len(os.sched_getaffinity(0))
+    # worker pools stay within the CPU share of one GPU (16 threads)
+    cores = max(1, min(affinity, 16))
+    flop_tuple = 12.0 * no ** 3 * (no + nv)
+    c1 = {}
+    t1 = O.synthetic(10, 50, 1)
+    for label, th in (("1_core", 1), ("all_cores", cores)):
+        dt, e = _timed_oracle(O, 10, 50, t1, 0, n_tuples(50), th)
+        c1[label] = {"threads": th, "seconds": dt, "tuples_per_s": n_tuples(50) / dt,
+                     "energy_Eh": e}
+    del t1
     t0 = time.time()
     t = O.synthetic(no, nv, seed, nthreads=cores)
     gen_s = time.time() - t0
-    flop_tuple = 12.0 * no ** 3 * (no + nv)
-    # calibrate on a small sample, then size the measured sample to the budget
-    probe = max(cores, 4)
-    t0 = time.time()
-    O.run(no, nv, t, first=0, count=probe, use_gemm=True, nthreads=cores)
-    dt = max(time.time() - t0, 1e-3)
-    count = int(max(probe, min(n_tuples(nv), probe * budget_s / dt)))
-    count = (count // cores) * cores or cores
-    t0 = time.time()
-    O.run(no, nv, t, first=0, count=count, use_gemm=True, nthreads=cores)
-    dt = time.time() - t0
-    rate = count / dt
+    samples = {}
+    for label, th in (("1_core", 1), ("all_cores", cores)):
+        count, dt, setup = _sized_sample(O, no, nv, t, th, budget_s)
+        rate = count / dt
+        samples[label] = {
+            "threads": th, "tuples": count, "seconds": dt, "setup_s": setup,
+            "tuples_per_s": rate,
+            "tflops": rate * flop_tuple / 1e12,
+            "extrapolated_full_space_s": n_tuples(nv) / rate,
+        }
+    del t
+    rate = samples["all_cores"]["tuples_per_s"]
     return {
         "value": rate,
         "unit": "tuples/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"first {count} tuples (lexicographic order) of No={no} "
-                   f"Nv={nv} seed={seed}; oracle port with dgemm from "
-                   f"{blas or 'plain C'}; {cores} tuple-shard threads, "
-                   f"single-threaded BLAS each; {dt:.1f} s measured, "
-                   f"{gen_s:.1f} s host generation untimed"),
+        "sample": (f"No={no} Nv={nv} seed={seed}: the first "
+                   f"{samples['all_cores']['tuples']} tuples in lexicographic order "
+                   f"({samples['all_cores']['seconds']:.1f} s) on {cores} tuple-shard "
+                   f"threads, extrapolated linearly to the {n_tuples(nv)} tuples "
+                   f"(per-tuple cost is uniform; the run's one-time slice setup "
+                   f"excluded); oracle port with dgemm from "
+                   f"{blas or 'plain C'}, single-threaded BLAS per thread; "
+                   f"{gen_s:.1f} s host tensor generation untimed"),
         "tflops": rate * flop_tuple / 1e12,
+        "cpu_model": cpu_model(),
+        "affinity_cpus": affinity,
+        "note": ("all-core runs use min(affinity, 16) threads, the host CPU "
+                 "share of one GPU"),
+        "one_core": samples["1_core"],
+        "all_cores": samples["all_cores"],
+        "c1_full": c1,
     }
 
 
@@ -190,18 +256,26 @@ def bench_serving(ctx, n_requests: int, rate: float, seed: int,
     return out
 
 
-def bench_c3(ctx, no: int, nv: int, seed: int, a0: int, a1: int, window: int):
-    """Plain FP64 vs the mixed forms at No=60 Nv=600 on the tuples whose first
-    index is in [a0, a1) (whole windows), each form on its own resident copy
-    (FP64 operands; split-only; FP32-only; bf16x3-only)."""
-    first = tuples_before(a0, nv)
-    count = tuples_before(a1, nv) - first
+C3_ROWS = None  # window rows (block pairs) at No=60 Nv=600, window 8
+
+
+def bench_c3(ctx, no: int, nv: int, seed: int, stride: int, window: int,
+             scale: float):
+    """Plain FP64 against the mixed forms at No=60 Nv=600 on a stratified
+    sample: every stride-th row of windows (all windows of one block pair
+    (alpha, beta)), spanning the whole first-index range.  Each form runs on
+    its own resident copy (FP64 operands; split-only; FP32-only;
+    bf16x3-only).  Then the same sample with every amplitude and integral
+    times `scale` (full-space E(T) of molecular magnitude) in FP64, bf16x3
+    and split FP32: the absolute |dE| at molecular size."""
+    nblk = (nv + window - 1) // window
+    rows = nblk * (nblk + 1) // 2
     flop_tuple = 12.0 * no ** 3 * (no + nv)
-    out = {"workload": (f"(T) No={no} Nv={nv} seed {seed}, tuples with first index "
-                        f"in [{a0},{a1}) ({count})"), "tuples": count}
+    out = {}
     ref = None
+    tuples = None
     for prec, store in (("fp64", 0), ("split_fp32", 1), ("fp32", 2), ("bf16x3", 3)):
-        log(f"No={no} Nv={nv} sample, {prec}")
+        log(f"No={no} Nv={nv} stratified sample, {prec}")
         ctx.set_option("operand_store", store)
         try:
             t0 = time.time()
@@ -210,22 +284,24 @@ def bench_c3(ctx, no: int, nv: int, seed: int, a0: int, a1: int, window: int):
         finally:
             ctx.set_option("operand_store", 0)
         try:
-            ctx.run(s_, prec, first=first, count=min(count, 1))  # warm
+            ctx.run(s_, prec, first=0, count=1)  # warm
             kname = KERNEL_NAME[prec]
             ctx.reset_stats()
             ctx.set_option("timing", 1)
             t0 = time.perf_counter()
-            e, _ = ctx.run(s_, prec, first=first, count=count)
+            e, _, nt = ctx.run_windows(s_, stride, 0, prec)
             dt = time.perf_counter() - t0
             ctx.set_option("timing", 0)
             k = ctx.kernel_stats(kname)
             info = ctx.sys_info(s_)
         finally:
             ctx.release(s_)
-        ent = {"value": count / dt, "unit": "tuples/s",
-               "tflops": count * flop_tuple / dt / 1e12,
+        tuples = nt
+        ent = {"value": nt / dt, "unit": "tuples/s",
+               "tflops": nt * flop_tuple / dt / 1e12,
                "doubles_kernel_tflops": k.flop / (k.total_ms * 1e-3) / 1e12 if k.total_ms else None,
-               "energy_Eh": e, "resident_gb": info["bytes"] / 1e9, "upload_s": up}
+               "energy_Eh": e, "resident_gb": info["bytes"] / 1e9, "upload_s": up,
+               "seconds": dt}
         if k.total_ms:
             ent["doubles_kernel_mfma_frac"] = (ent["doubles_kernel_tflops"] * MFMA_PER_FLOP[prec]
                                                / PEAK_TFLOPS[prec])
@@ -236,6 +312,37 @@ def bench_c3(ctx, no: int, nv: int, seed: int, a0: int, a1: int, window: int):
             ent["abs_dE_Eh"] = abs(e - ref["energy_Eh"])
             ent["rel_dE"] = abs(e - ref["energy_Eh"]) / abs(ref["energy_Eh"])
         out[prec] = ent
+    out["workload"] = (f"(T) No={no} Nv={nv} seed {seed}: stratified sample, every "
+                       f"{stride}-th of the {rows} rows of windows (block pairs "
+                       f"(alpha,beta) of edge {window}), spanning a in [0,{nv}): "
+                       f"{tuples} tuples = {tuples / n_tuples(nv):.2%} of {n_tuples(nv)}")
+    out["tuples"] = tuples
+    out["fraction"] = tuples / n_tuples(nv)
+    scaled = {"scale": scale}
+    e64 = None
+    for prec, store in (("fp64", 0), ("bf16x3", 3), ("split_fp32", 1)):
+        log(f"No={no} Nv={nv} stratified sample, inputs x {scale}, {prec}")
+        ctx.set_option("operand_store", store)
+        try:
+            s_ = ctx.upload_synthetic_scaled(no, nv, seed, scale)
+        finally:
+            ctx.set_option("operand_store", 0)
+        try:
+            e, _, _ = ctx.run_windows(s_, stride, 0, prec)
+        finally:
+            ctx.release(s_)
+        if e64 is None:
+            e64 = e
+            scaled["fp64_energy_Eh"] = e
+            scaled["scaling_law_rel_err"] = (abs(e - ref["energy_Eh"] * scale ** 4)
+                                             / abs(ref["energy_Eh"] * scale ** 4))
+        else:
+            scaled[prec] = {"energy_Eh": e, "abs_dE_Eh": abs(e - e64),
+                            "rel_dE": abs(e - e64) / abs(e64)}
+    scaled["note"] = ("amplitudes and integrals times the scale: the full-space "
+                      "E(T) of this shape is then about scale^4 x the unscaled "
+                      "one (0.5 Eh order); |dE| is measured on the sample")
+    out["molecular_scale"] = scaled
     return out
 
 
@@ -254,7 +361,8 @@ def main(argv=None):
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--slices", type=int, default=16,
                     help="steps that cover the tuple space once")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="length of each CPU-baseline sample of the GPU workload")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision comparison pass")
@@ -266,8 +374,14 @@ def main(argv=None):
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--no-c3", action="store_true",
-                    help="skip the No=60 Nv=600 FP64-vs-mixed sample")
-    ap.add_argument("--c3-a0", type=int, default=400)
+                    help="skip the No=60 Nv=600 FP64-vs-mixed stratified sample")
+    ap.add_argument("--c3-stride", type=int, default=55,
+                    help="every n-th row of windows in the No=60 Nv=600 sample")
+    ap.add_argument("--c3-scale", type=float, default=4e-3,
+                    help="amplitude/integral scale of the molecular-size check")
+    ap.add_argument("--step-offset", type=int, default=0,
+                    help="first slice of the timed steps (to split one pass of "
+                         "the tuple space over several runs)")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -292,15 +406,21 @@ def main(argv=None):
         first, count = slices[(len(slices) - 1 - i) % len(slices)]
         ctx.run(sys_, args.precision, first=first, count=count)
 
+    def step_range(i):
+        return slices[(args.step_offset + i) % len(slices)]
+
     ctx.reset_stats()
     ctx.set_option("timing", 1)
     done = 0
     e_sum = 0.0
+    pass_e = []  # energies of the first pass over the slices
     t_start = time.perf_counter()
     for i in range(args.steps):
-        first, count = slices[i % len(slices)]
+        first, count = step_range(i)
         e, _ = ctx.run(sys_, args.precision, first=first, count=count)
         e_sum += e
+        if i < len(slices):
+            pass_e.append(e)
         done += count
         log(f"step {i + 1}/{args.steps}: {count} tuples")
     t_end = time.perf_counter()
@@ -328,7 +448,7 @@ def main(argv=None):
             m_sum = 0.0
             t0m = time.perf_counter()
             for i in range(args.steps):
-                first, count = slices[i % len(slices)]
+                first, count = step_range(i)
                 em, _ = ctx.run(sys_, form, first=first, count=count)
                 m_sum += em
                 log(f"{form} step {i + 1}/{args.steps}")
@@ -357,16 +477,26 @@ def main(argv=None):
     serving = None if args.no_serve else bench_serving(ctx, args.serve_requests, 20.0, 0)
     c3 = None
     if not args.no_c3:
-        c3 = bench_c3(ctx, 60, 600, 1, args.c3_a0, args.c3_a0 + args.window,
-                      args.window)
+        c3 = bench_c3(ctx, 60, 600, 1, args.c3_stride, args.window, args.c3_scale)
 
     cpu = None
     if not args.no_cpu:
         log("CPU baseline")
-        cores = min(len(os.sched_getaffinity(0)), 16)
-        cpu = cpu_baseline(args.no, args.nv, args.seed, args.cpu_seconds, cores)
+        cpu = cpu_baseline(args.no, args.nv, args.seed, args.cpu_seconds)
 
-    full = args.steps % len(slices) == 0 and args.steps > 0
+    # E(T) of the first pass over the slices: the whole tuple space when the
+    # pass started at slice 0 and completed, else the covered slices only
+    first_pass = None
+    if pass_e:
+        covered = sorted({(args.step_offset + i) % len(slices) for i in range(len(pass_e))})
+        first_pass = {"slices": [covered[0], covered[-1]] if covered else None,
+                      "of": len(slices), "energy_Eh": neumaier(pass_e),
+                      "whole_space": len(covered) == len(slices)}
+    traffic_kernel = "doubles_f64" if args.precision == "fp64" else "doubles_split"
+    traffic = measured_traffic(traffic_kernel, args.no, args.nv)
+    traffic_src = ("recorded rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of the "
+                   f"{traffic_kernel} kernel at No={args.no} Nv={args.nv}, read from "
+                   "profiles/ (not counted in this run)")
     line = {
         "metric": METRIC,
         "value": value,
@@ -395,19 +525,20 @@ def main(argv=None):
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": measured_traffic(
-                "doubles_f64" if args.precision == "fp64" else "doubles_split",
-                args.no, args.nv),
+            "traffic": traffic,
+            "traffic_source": (traffic_src if traffic is not None else None),
             "kernel": kname,
             "launches": ks.launches,
             "avg_launch_ms": ks.avg_ms,
         },
         "energy_kernel_ms": es.total_ms,
         "tuples_timed": done,
-        "energy_Eh": e_sum if full else None,
+        "energy_Eh": (first_pass["energy_Eh"]
+                      if first_pass and first_pass["whole_space"] else None),
+        "energy_first_pass": first_pass,
         "setup_s": gen_s,
         "mixed_precision": mixed,
-        "mixed_c3": c3,
+        "c3_stratified": c3,
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -158,6 +158,21 @@ int atrip_hip_run(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
                   uint64_t first, uint64_t count, double *energy,
                   double *ct_energy);
 
+/* (T) energy of a systematic sample of the tuple space.  The window plan
+ * cuts the virtual index into blocks of edge "window"; its windows are the
+ * block triples (alpha <= beta <= gamma) holding a tuple, and a row is all
+ * windows of one block pair (alpha, beta).  The sample is the rows offset,
+ * offset + stride, offset + 2 stride, ... in lexicographic order, every
+ * valid tuple of each; *tuples receives their number.  Rows keep the
+ * operand reuse of a contiguous run, and the sample spans the whole
+ * first-index range: for measuring throughput and precision-form
+ * differences on shapes whose full tuple space takes too long.  stride 1
+ * is the whole space.  No counterpart in the reference. */
+int atrip_hip_run_windows(atrip_hip_ctx *ctx, int sys, int precision,
+                          int ijkabc, uint64_t stride, uint64_t offset,
+                          double *energy, double *ct_energy,
+                          uint64_t *tuples);
+
 /* Per-tuple energies (raw, before the sign) for the tuples
  * [first, first+count) in lexicographic order; e_ct may be NULL.  The
  * reference keeps these only in a never-filled debug map
