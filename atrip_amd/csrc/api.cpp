@@ -673,10 +673,23 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   RunResult res;
   check_precision(s, precision);
   const bool g32 = precision != ATRIP_FP64;
-  if (precision == ATRIP_BF16X3)
+  if (fixed) {
+    // the operand copies of a mixed form are device allocations: a run
+    // without allocation needs them resident already
+    if (precision == ATRIP_BF16X3 && !s.tri_ready)
+      throw Fail{"bf16x3 operand copies are not resident: run the system "
+                 "once in bf16x3 through atrip_hip_run, or upload it with "
+                 "operand_store 3"};
+    if ((precision == ATRIP_SPLIT_FP32 || precision == ATRIP_FP32) &&
+        !s.split_ready)
+      throw Fail{"split FP32 operand copies are not resident: run the system "
+                 "once in a split form through atrip_hip_run, or upload it "
+                 "with operand_store 1 or 2"};
+  } else if (precision == ATRIP_BF16X3) {
     ensure_tri(ctx, s);
-  else if (g32)
+  } else if (g32) {
     ensure_split(ctx, s);
+  }
   if (first >= last) return res;
   const int nb = plan.nb;
   const bool full = first == 0 && last == n;
@@ -765,12 +778,22 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   if (fixed) {
     while (bw > 1 && total(sizes(bw)) > fixed->bytes) bw--;
     z = sizes(bw);
-    if (total(z) > fixed->bytes)
+    if (total(z) > fixed->bytes) {
+      // largest window edge whose single window fits (class buffers, cubes
+      // and slot energies scale with edge^3)
+      const size_t per_slot = (z.g + z.wcube + z.e_slot) / (size_t)nb3;
+      const size_t rest = a256(z.partials) + a256(z.windows) + 3 * 256;
+      int fit = 0;
+      for (int e = nb - 1; e >= 1 && !fit; e--)
+        if ((size_t)e * e * e * per_slot + rest <= fixed->bytes) fit = e;
       throw Fail{"run needs " + std::to_string(total(z)) +
                  " bytes of scratch for one window of edge " +
                  std::to_string(nb) + " at No=" + std::to_string(s.No) +
                  "; the allocation-free entry point holds " +
-                 std::to_string(fixed->bytes) + " (use a smaller window)"};
+                 std::to_string(fixed->bytes) +
+                 (fit ? " (window edge " + std::to_string(fit) + " fits)"
+                      : " (no window edge fits this No)")};
+    }
     char *p = (char *)fixed->base;
     Gbuf = p;
     p += a256(z.g);
@@ -947,6 +970,19 @@ struct NoSink {
 };
 
 }  // namespace
+
+void atrip_amd::run_fixed(atrip_hip_ctx *ctx, int sys, int precision,
+                          int ijkabc, uint64_t first, uint64_t count,
+                          void *base, size_t bytes, double *energy,
+                          double *ct_energy) {
+  const uint64_t last =
+      count == ~0ULL || first + count < first ? ~0ULL : first + count;
+  const FixedScratch fx{base, bytes};
+  RunResult r = run_range(ctx, sys, precision, ijkabc, first, last, NoSink{},
+                          1, 0, &fx);
+  *energy = ijkabc ? r.e : -r.e;
+  if (ct_energy) *ct_energy = ijkabc ? r.ect : -r.ect;
+}
 
 extern "C" {
 

@@ -57,6 +57,16 @@ void check_precision(const DevSystem &s, int precision);
 // store mode `store` (what admission charges).
 uint64_t system_bytes(int64_t No, int64_t Nv, bool with_j, int store);
 
+// atrip_hip_run over [first, first + count) with every piece of device
+// scratch (class buffers, W cubes, slot energies, batch partials, window
+// list) carved out of [base, base + bytes): no device allocation, no timing
+// events.  The windows per launch shrink until a launch fits; a run whose
+// single window does not fit, or a mixed form whose operand copies are not
+// resident yet, throws Fail.  Writes the signed energies like atrip_hip_run.
+void run_fixed(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
+               uint64_t first, uint64_t count, void *base, size_t bytes,
+               double *energy, double *ct_energy);
+
 // Number of tuples a <= b <= c < Nv, not all equal.
 inline uint64_t n_tuples(uint64_t Nv) {
   return Nv * (Nv + 1) * (Nv + 2) / 6 - Nv;

@@ -36,6 +36,7 @@ SYMBOLS = (
     "atrip_hip_serve_set_policy", "atrip_hip_serve_clock_ms",
     "atrip_hip_serve_submit_synthetic", "atrip_hip_serve_submit",
     "atrip_hip_serve_step", "atrip_hip_serve_result", "atrip_hip_serve_stats",
+    "atrip_hip_run_noalloc",
 )
 
 TENSORS = ("ei", "ea", "Tai", "Tabij", "Vabij", "Vijka", "Vabci", "Jijka", "Jabci")
@@ -91,6 +92,9 @@ def load():
     L.atrip_hip_upload_synthetic_complex.argtypes = [
         vp, C.c_int64, C.c_int64, C.c_uint64, C.c_int, C.POINTER(C.c_int)]
     L.atrip_hip_run.argtypes = [
+        vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
+        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.atrip_hip_run_noalloc.argtypes = [
         vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
         C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.atrip_hip_run_windows.argtypes = [
@@ -302,6 +306,17 @@ class Context:
         self._check(self._L.atrip_hip_run(
             self._h, sys_, PRECISIONS[precision], int(ijkabc), first, cnt,
             C.byref(e), C.byref(ect)), "run")
+        return e.value, ect.value
+
+    def run_noalloc(self, sys_: int, precision="fp64", ijkabc=False, first=0,
+                    count=None):
+        """run() with every piece of device scratch in the library's fixed
+        per-GPU region (no device allocation)."""
+        e, ect = C.c_double(), C.c_double()
+        cnt = (1 << 64) - 1 if count is None else count
+        self._check(self._L.atrip_hip_run_noalloc(
+            self._h, sys_, PRECISIONS[precision], int(ijkabc), first, cnt,
+            C.byref(e), C.byref(ect)), "run_noalloc")
         return e.value, ect.value
 
     def run_windows(self, sys_: int, stride: int, offset: int = 0, precision="fp64",

@@ -158,6 +158,30 @@ int atrip_hip_run(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
                   uint64_t first, uint64_t count, double *energy,
                   double *ct_energy);
 
+/* Bytes of the fixed device region of atrip_hip_run_noalloc (per GPU). */
+#define ATRIP_NOALLOC_SCRATCH_BYTES (256ull * 1024ull * 1024ull)
+
+/* atrip_hip_run without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and sign as atrip_hip_run; per-tuple energies are
+ * bitwise those of atrip_hip_run, and E(T) is the same sum when the same
+ * windows per launch fit (else it agrees to rounding).  All device scratch
+ * (class buffers, W cubes, slot energies, batch partials, window list)
+ * comes from one fixed region of ATRIP_NOALLOC_SCRATCH_BYTES per GPU, held
+ * from the first call on that GPU until the process ends; the windows per
+ * launch shrink until a launch fits.  Fails with a clear error, before any
+ * launch, when one window of the context's "window" edge needs more
+ * (FP64 without (cT): No=40 fits edge 5, No=60 edge 3; the message names
+ * the largest edge that fits), and for a mixed form whose operand copies are not
+ * resident yet (upload with that operand_store, or run the form once
+ * through atrip_hip_run).  No timing events are recorded.  Calls share the
+ * GPU's region: calls that overlap on one GPU (two threads, or contexts
+ * on different streams of one device) race and give wrong results. */
+int atrip_hip_run_noalloc(atrip_hip_ctx *ctx, int sys, int precision,
+                          int ijkabc, uint64_t first, uint64_t count,
+                          double *energy, double *ct_energy);
+
 /* (T) energy of a systematic sample of the tuple space.  The window plan
  * cuts the virtual index into blocks of edge "window"; its windows are the
  * block triples (alpha <= beta <= gamma) holding a tuple, and a row is all
