@@ -660,10 +660,18 @@ struct FixedScratch {
   size_t bytes;
 };
 
+// Which windows of the plan a run covers besides the tuple range: every
+// stride-th row of windows (block pairs) from row offset, and plan windows
+// [wbegin, wend) by their index in the plan.
+struct WindowSel {
+  uint64_t stride = 1, offset = 0;
+  uint64_t wbegin = 0, wend = ~0ULL;
+};
+
 template <typename Sink>
 RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
                     uint64_t first, uint64_t last, Sink &&sink,
-                    uint64_t wstride = 1, uint64_t woffset = 0,
+                    const WindowSel &wsel = WindowSel{},
                     const FixedScratch *fixed = nullptr) {
   drop_pending_events(ctx);
   DevSystem &s = get_sys(ctx, sys);
@@ -707,7 +715,9 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
       prev_al = al;
       prev_be = be;
     }
-    if (wstride > 1 && (uint64_t)pair % wstride != woffset % wstride) continue;
+    if (w < wsel.wbegin || w >= wsel.wend) continue;
+    if (wsel.stride > 1 && (uint64_t)pair % wsel.stride != wsel.offset % wsel.stride)
+      continue;
     uint64_t cnt = 0;
     if (full || (plan.wmin[w] >= first && plan.wmax[w] < last)) {
       cnt = plan.wcount[w];
@@ -732,8 +742,9 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   const int nsel = (int)counts.size();
   if (!nsel) return res;
   // Device scratch of the run: class buffers G (and those of the (cT)
-  // pass; four sets for complex systems), per-slot energies, W cubes, batch
-  // partials and the window list.  Taken from the context's buffers (grown
+  // pass; four sets for complex systems), per-slot energies, the energy
+  // kernels' scratch (per-triple partials, or W cubes for complex systems
+  // and the cube epilogue), batch partials and the window list.  Taken from the context's buffers (grown
   // on demand), or carved out of a fixed region without any allocation
   // (then the windows per launch shrink until the run fits).
   const int64_t nb3 = (int64_t)nb * nb * nb;
@@ -741,6 +752,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   const bool use_j = s.has_j;
   const int nsets = s.cplx ? 4 : 1;
   const size_t gelem = g32 ? sizeof(float) : sizeof(double);
+  const bool fused = ctx->epilogue == 0;
   struct Sizes {
     int64_t slots_max = 0;
     size_t gsz = 0, g = 0, e_slot = 0, wcube = 0, partials = 0, windows = 0;
@@ -752,8 +764,8 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
     z.gsz = (size_t)(3 * z.slots_max * W3) * gelem;
     z.g = (size_t)((use_j ? 2 : 1) * nsets) * z.gsz;
     z.e_slot = (size_t)(2 * z.slots_max) * sizeof(double);
-    z.wcube = (size_t)((use_j ? 2 : 1) * (s.cplx ? 2 : 1) * z.slots_max *
-                       w_cube(s.No)) *
+    z.wcube = (size_t)(z.slots_max * energy_scratch_per_slot(
+                                         s.No, use_j, s.cplx, fused)) *
               sizeof(double);
     z.nbatch = (nsel + bw_ - 1) / bw_;
     z.partials = (size_t)(4 * z.nbatch) * sizeof(double);
@@ -765,6 +777,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
     return a256(z.g) + a256(z.e_slot) + a256(z.wcube) + a256(z.partials) +
            a256(z.windows);
   };
+
   int bw = std::max(1, ctx->batch);
   void *Gbuf = nullptr, *e_slot = nullptr, *wcube = nullptr,
        *partials = nullptr, *d_sel = nullptr;
@@ -815,7 +828,8 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
           "upload windows");
     ensure((void *&)ctx->G, ctx->G_bytes, z.g, "alloc G");
     ensure((void *&)ctx->e_slot, ctx->e_slot_bytes, z.e_slot, "alloc e_slot");
-    ensure((void *&)ctx->wcube, ctx->wcube_bytes, z.wcube, "alloc W cubes");
+    ensure((void *&)ctx->wcube, ctx->wcube_bytes, z.wcube,
+           "alloc energy scratch");
     ensure((void *&)ctx->partials, ctx->partials_bytes, z.partials,
            "alloc partials");
     Gbuf = ctx->G;
@@ -827,15 +841,8 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   const int nbatch = z.nbatch;
   DevSystem zpass[4];
   if (s.cplx) complex_passes(s, zpass);
-  double *Gz[4], *GJz[4];
-  for (int q = 0; q < 4; q++) {
-    Gz[q] = (double *)((char *)Gbuf + q * gsz);
-    GJz[q] = (double *)((char *)Gbuf + (4 + q) * gsz);
-  }
   // timing events are objects the runtime creates: none without allocation
   const bool timing = ctx->timing && !fixed;
-  void *G = Gbuf;
-  void *GJ = use_j ? (void *)((char *)Gbuf + gsz) : nullptr;
   // complex products are four real ones (the reference bench counts 4x)
   const double flop_tuple = 12.0 * (double)s.No * s.No * s.No * (s.No + s.Nv) *
                             (s.cplx ? 4.0 : 1.0);
@@ -843,6 +850,14 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
                       : precision == ATRIP_SPLIT_FP32 ? "doubles_mixed"
                       : precision == ATRIP_BF16X3     ? "doubles_bf16x3"
                                                       : "doubles_fp32";
+  hipStream_t st = ctx->stream;
+  double *Gz[4], *GJz[4];
+  for (int q = 0; q < 4; q++) {
+    Gz[q] = (double *)((char *)Gbuf + q * gsz);
+    GJz[q] = (double *)((char *)Gbuf + (4 + q) * gsz);
+  }
+  void *G = Gbuf;
+  void *GJ = use_j ? (void *)((char *)Gbuf + gsz) : nullptr;
   std::vector<double> host_slots;
 
   for (int bi = 0; bi < nbatch; bi++) {
@@ -867,73 +882,67 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
             if (x) (void)hipEventDestroy(x);
       }
     } ev;
-    hipEvent_t &e0 = ev.e[0], &e1 = ev.e[1], &e2 = ev.e[2];
     if (timing) {
-      check(hipEventCreate(&e0), "event");
-      check(hipEventCreate(&e1), "event");
-      check(hipEventCreate(&e2), "event");
-      check(hipEventRecord(e0, ctx->stream), "event");
+      for (hipEvent_t &x : ev.e) check(hipEventCreate(&x), "event");
+      check(hipEventRecord(ev.e[0], st), "event");
     }
     for (int pass = 0; pass < (use_j ? 2 : 1) && s.cplx; pass++)
       for (int q = 0; q < 4; q++)
         check(launch_doubles_f64(zpass[q], b, pass == 1,
-                                 pass == 0 ? Gz[q] : GJz[q], ctx->tile,
-                                 ctx->stream),
+                                 pass == 0 ? Gz[q] : GJz[q], ctx->tile, st),
               "doubles kernel (complex pass)");
     for (int pass = 0; pass < (use_j ? 2 : 1) && !s.cplx; pass++) {
       void *Gp = pass == 0 ? G : GJ;
       if (precision == ATRIP_FP64)
-        check(launch_doubles_f64(s, b, pass == 1, (double *)Gp, ctx->tile,
-                                 ctx->stream),
+        check(launch_doubles_f64(s, b, pass == 1, (double *)Gp, ctx->tile, st),
               "doubles kernel");
       else if (precision == ATRIP_BF16X3)
-        check(launch_doubles_tri(s, b, pass == 1, (float *)Gp, ctx->tile,
-                                 ctx->stream),
+        check(launch_doubles_tri(s, b, pass == 1, (float *)Gp, ctx->tile, st),
               "doubles (bf16x3) kernel");
       else
         check(launch_doubles_split(s, b, pass == 1, (float *)Gp,
                                    precision == ATRIP_SPLIT_FP32, ctx->tile,
-                                   ctx->stream),
+                                   st),
               "doubles (FP32 products) kernel");
     }
-    if (timing) check(hipEventRecord(e1, ctx->stream), "event");
+    if (timing) check(hipEventRecord(ev.e[1], st), "event");
     if (s.cplx)
       check(launch_energy_z(s, b, Gz, use_j ? GJz : nullptr,
-                            ijkabc ? -1.0 : 1.0, (double *)e_slot, (double *)wcube,
-                            ctx->stream),
+                            ijkabc ? -1.0 : 1.0, (double *)e_slot,
+                            (double *)wcube, st),
             "energy kernel (complex)");
     else
-      check(launch_energy(s, b, G, GJ, g32, ijkabc ? -1.0 : 1.0, (double *)e_slot,
-                          (double *)wcube, ctx->stream),
+      check(launch_energy(s, b, G, GJ, g32, ijkabc ? -1.0 : 1.0,
+                          (double *)e_slot, (double *)wcube, fused, st),
             "energy kernel");
     if (timing) {
-      check(hipEventRecord(e2, ctx->stream), "event");
-      ctx->pending_events.push_back({e0, e1});
+      check(hipEventRecord(ev.e[2], st), "event");
+      ctx->pending_events.push_back({ev.e[0], ev.e[1]});
       ctx->pending_names.push_back(dname);
       ctx->pending_flop.push_back(flop_tuple * cnt * (use_j ? 2 : 1));
-      ctx->pending_events.push_back({e1, e2});
+      ctx->pending_events.push_back({ev.e[1], ev.e[2]});
       ctx->pending_names.push_back("energy");
       ctx->pending_flop.push_back(0.0);
       ev.handed = true;
     }
     check(launch_batch_reduce(s, b, (double *)e_slot, (double *)partials, bi,
-                              ctx->stream),
+                              st),
           "reduce kernel");
     if (sink.active()) {
       host_slots.resize(2 * b.slots);
       check(hipMemcpyAsync(host_slots.data(), (double *)e_slot,
                            host_slots.size() * sizeof(double),
-                           hipMemcpyDeviceToHost, ctx->stream),
+                           hipMemcpyDeviceToHost, st),
             "download slots");
-      check(hipStreamSynchronize(ctx->stream), "slots");
+      check(hipStreamSynchronize(st), "slots");
       sink(sel.data() + 3 * w0, b.nw, nb, host_slots);
     }
   }
   std::vector<double> part(4 * nbatch);
   check(hipMemcpyAsync(part.data(), (double *)partials, part.size() * sizeof(double),
-                       hipMemcpyDeviceToHost, ctx->stream),
+                       hipMemcpyDeviceToHost, st),
         "download partials");
-  check(hipStreamSynchronize(ctx->stream), "run");
+  check(hipStreamSynchronize(st), "run");
   // Neumaier over the batch partials in batch order
   double s0 = 0, c0 = 0, s1 = 0, c1 = 0;
   auto add = [](double &sum, double &comp, double x) {
@@ -971,6 +980,40 @@ struct NoSink {
 
 }  // namespace
 
+uint64_t atrip_amd::plan_windows(atrip_hip_ctx *ctx, int sys) {
+  DevSystem &s = get_sys(ctx, sys);
+  return get_plan(ctx, sys, s).windows.size() / 3;
+}
+
+void atrip_amd::run_plan_windows(atrip_hip_ctx *ctx, int sys, int precision,
+                                 uint64_t wbegin, uint64_t wend,
+                                 std::vector<double> &out) {
+  out.clear();
+  struct WinSink {
+    std::vector<double> *out;
+    bool active() const { return true; }
+    void operator()(const int32_t *, int nw, int nb,
+                    const std::vector<double> &slots) {
+      const int nb3 = nb * nb * nb;
+      for (int w = 0; w < nw; w++) {
+        double sum = 0.0, comp = 0.0;
+        for (int q = 0; q < nb3; q++) {
+          const double x = slots[2 * ((size_t)w * nb3 + q)];
+          const double t = sum + x;
+          comp += (std::fabs(sum) >= std::fabs(x)) ? (sum - t) + x : (x - t) + sum;
+          sum = t;
+        }
+        out->push_back(sum);
+        out->push_back(comp);
+      }
+    }
+  } sink{&out};
+  WindowSel ws;
+  ws.wbegin = wbegin;
+  ws.wend = wend;
+  run_range(ctx, sys, precision, 0, 0, ~0ULL, sink, ws);
+}
+
 void atrip_amd::run_fixed(atrip_hip_ctx *ctx, int sys, int precision,
                           int ijkabc, uint64_t first, uint64_t count,
                           void *base, size_t bytes, double *energy,
@@ -979,7 +1022,7 @@ void atrip_amd::run_fixed(atrip_hip_ctx *ctx, int sys, int precision,
       count == ~0ULL || first + count < first ? ~0ULL : first + count;
   const FixedScratch fx{base, bytes};
   RunResult r = run_range(ctx, sys, precision, ijkabc, first, last, NoSink{},
-                          1, 0, &fx);
+                          WindowSel{}, &fx);
   *energy = ijkabc ? r.e : -r.e;
   if (ct_energy) *ct_energy = ijkabc ? r.ect : -r.ect;
 }
@@ -1208,8 +1251,10 @@ int atrip_hip_run_windows(atrip_hip_ctx *ctx, int sys, int precision,
     if (!energy) throw Fail{"null energy pointer"};
     if (stride == 0) throw Fail{"stride must be positive"};
     check(hipSetDevice(ctx->device), "hipSetDevice");
-    RunResult r = run_range(ctx, sys, precision, ijkabc, 0, ~0ULL, NoSink{},
-                            stride, offset);
+    WindowSel ws;
+    ws.stride = stride;
+    ws.offset = offset;
+    RunResult r = run_range(ctx, sys, precision, ijkabc, 0, ~0ULL, NoSink{}, ws);
     *energy = ijkabc ? r.e : -r.e;
     if (ct_energy) *ct_energy = ijkabc ? r.ect : -r.ect;
     if (tuples) *tuples = r.tuples;
@@ -1271,6 +1316,13 @@ int atrip_hip_set_option(atrip_hip_ctx *ctx, const char *key, int64_t value) {
       if (value < 0 || value > 3)
         throw Fail{"operand_store must be 0, 1, 2 or 3"};
       ctx->operand_store = (int)value;
+    } else if (k == "serve_plain_no") {
+      if (value < 1 || value > (1 << 20))
+        throw Fail{"serve_plain_no must be in 1..1048576"};
+      ctx->serve_plain_no = (int)value;
+    } else if (k == "epilogue") {
+      if (value < 0 || value > 1) throw Fail{"epilogue must be 0 or 1"};
+      ctx->epilogue = (int)value;
     } else if (k == "stage_kb") {
       if (value < 1 || value > ((int64_t)1 << 30))
         throw Fail{"stage_kb must be in 1..2^30"};
@@ -1310,6 +1362,8 @@ int atrip_hip_get_option(atrip_hip_ctx *ctx, const char *key, int64_t *value) {
     else if (k == "tile") *value = ctx->tile;
     else if (k == "operand_store") *value = ctx->operand_store;
     else if (k == "stage_kb") *value = ctx->stage_kb;
+    else if (k == "epilogue") *value = ctx->epilogue;
+    else if (k == "serve_plain_no") *value = ctx->serve_plain_no;
     else if (k == "batch_scratch_mb") *value = ctx->batch_scratch_mb;
     else if (k == "batch_grid") *value = ctx->batch_grid;
     else if (k == "batch_order") *value = ctx->batch_order;

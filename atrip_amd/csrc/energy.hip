@@ -153,6 +153,97 @@ __device__ __forceinline__ void orbit(int B1, int B2, int B3, int nbk,
   }
 }
 
+// Energy terms of the i>=j>=k points of block triple (B1, B2, B3) (edge 8)
+// of one tuple, two points per thread, added to e_acc (and, with the (cT)
+// W, to ect_acc).  The six permuted W cubes of the triple's orbit sit in
+// LDS (cube[alias[p]] holds W at permutation p, element x + 8 y + CZ z), as
+// do the singles tables tA/tB/tC (Tai at (a,x), (b,x), (c,x), slot 8 * position
+// + local index), the eps_i table sE and the V^bc, V^ac, V^ab 8x8 block pairs
+// sV.  Z = W + singles (reference src/atrip/Equations.cxx:387-426) and the
+// get_energy_distinct / get_energy_same formulas (:101-238) in the
+// reference's expression order.
+template <bool HJ, int CZ, int CSZ>
+__device__ __forceinline__ void triple_points(
+    int B1, int B2, int B3, int no, int distinct, double epsabc,
+    const int (&alias)[6], const double (*cube)[CSZ], const double *cubeJ,
+    const double *tA, const double *tB, const double *tC, const double *sE,
+    const double (*sV)[3][3][64], int tid, double &e_acc, double &ect_acc) {
+  // both points of a thread in one body, so their LDS reads and FP64
+  // chains interleave
+#pragma unroll
+  for (int pt = tid; pt < 512; pt += 256) {
+    const int li = pt & 7, lj = (pt >> 3) & 7, lk = pt >> 6;
+    const int i = B1 * 8 + li, j = B2 * 8 + lj, k = B3 * 8 + lk;
+    if (i >= no || j >= no || k >= no || i < j || j < k) continue;
+    const double ei = sE[li], ej = sE[8 + lj], ek = sE[16 + lk];
+    const double facjk = j == k ? 0.5 : 1.0;
+    const double facij = i == j ? 0.5 : 1.0;
+    const double eijk = (ei + ej) + ek;
+    const double denominator = epsabc - eijk;
+    // local offsets of the permutations inside their cubes
+    const int o_ijk = li + 8 * lj + CZ * lk, o_ikj = li + 8 * lk + CZ * lj,
+              o_jik = lj + 8 * li + CZ * lk, o_jki = lj + 8 * lk + CZ * li,
+              o_kij = lk + 8 * li + CZ * lj, o_kji = lk + 8 * lj + CZ * li;
+    // singles at a permutation (x,y,z): t_a[x] V^bc_yz + t_b[y] V^ac_xz
+    // + t_c[z] V^ab_xy; slot s = 8 * position + local index (position 0:
+    // i, 1: j, 2: k)
+    auto Z = [&](double wv, int sx, int sy, int sz) {
+      const int px = sx >> 3, py = sy >> 3, pz = sz >> 3;
+      const int lx = sx & 7, ly = sy & 7, lz = sz & 7;
+      double zz = wv + tA[sx] * sV[0][py][pz][ly + 8 * lz];
+      zz = zz + tB[sy] * sV[1][px][pz][lx + 8 * lz];
+      zz = zz + tC[sz] * sV[2][px][py][lx + 8 * ly];
+      return zz;
+    };
+    auto J = [&](int p, int o) { return cubeJ[alias[p] * CSZ + o]; };
+    const int si = li, sj = 8 + lj, sk = 16 + lk;
+    if (distinct == 0) {
+      const double A = cube[alias[0]][o_ijk], B = cube[alias[1]][o_ikj],
+                   C = cube[alias[2]][o_jik], D = cube[alias[3]][o_jki],
+                   E = cube[alias[4]][o_kij], F = cube[alias[5]][o_kji];
+      const double U = Z(A, si, sj, sk), V = Z(B, si, sk, sj),
+                   W = Z(C, sj, si, sk), X = Z(D, sj, sk, si),
+                   Y = Z(E, sk, si, sj), Zz = Z(F, sk, sj, si);
+      const double UXY = U + (X + Y), VWZ = V + (W + Zz);
+      {
+        const double first =
+            A * U + (B * V + (C * W + (D * X + (E * Y + F * Zz))));
+        const double second = (UXY - 2.0 * VWZ) * (A + (D + E));
+        const double third = (VWZ - 2.0 * UXY) * (B + (C + F));
+        const double value = 3.0 * first + (second + third);
+        e_acc += ((2.0 * value) / denominator) * (facjk * facij);
+      }
+      if constexpr (HJ) {
+        const double A2 = J(0, o_ijk), B2 = J(1, o_ikj), C2 = J(2, o_jik),
+                     D2 = J(3, o_jki), E2 = J(4, o_kij), F2 = J(5, o_kji);
+        const double first =
+            A2 * U + (B2 * V + (C2 * W + (D2 * X + (E2 * Y + F2 * Zz))));
+        const double second = (UXY - 2.0 * VWZ) * (A2 + (D2 + E2));
+        const double third = (VWZ - 2.0 * UXY) * (B2 + (C2 + F2));
+        const double value = 3.0 * first + (second + third);
+        ect_acc += ((2.0 * value) / denominator) * (facjk * facij);
+      }
+    } else {
+      const double A = cube[alias[0]][o_ijk], B = cube[alias[3]][o_jki],
+                   C = cube[alias[4]][o_kij];
+      const double U = Z(A, si, sj, sk), V = Z(B, sj, sk, si),
+                   W = Z(C, sk, si, sj);
+      const double UVW = U + (V + W);
+      {
+        const double value =
+            3.0 * ((A * U + B * V) + C * W) - (A + (B + C)) * UVW;
+        e_acc += ((2.0 * value) / denominator) * (facjk * facij);
+      }
+      if constexpr (HJ) {
+        const double A2 = J(0, o_ijk), B2 = J(3, o_jki), C2 = J(4, o_kij);
+        const double value =
+            3.0 * ((A2 * U + B2 * V) + C2 * W) - (A2 + (B2 + C2)) * UVW;
+        ect_acc += ((2.0 * value) / denominator) * (facjk * facij);
+      }
+    }
+  }
+}
+
 template <bool HJ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void energy_kernel(EnergyParams P) {
@@ -309,83 +400,9 @@ void energy_kernel(EnergyParams P) {
       int cb_[6], nc_;
       orbit(B1, B2, B3, nbk, cb_, nc_, alias);
     }
-    {
-      // both points of a thread in one body, so their LDS reads and FP64
-      // chains interleave
-#pragma unroll
-        for (int pt = tid; pt < 512; pt += 256) {
-          const int li = pt & 7, lj = (pt >> 3) & 7, lk = pt >> 6;
-          const int i = B1 * 8 + li, j = B2 * 8 + lj, k = B3 * 8 + lk;
-          if (i >= no || j >= no || k >= no || i < j || j < k) continue;
-          const double ei = sE[li], ej = sE[8 + lj], ek = sE[16 + lk];
-          const double facjk = j == k ? 0.5 : 1.0;
-          const double facij = i == j ? 0.5 : 1.0;
-          const double eijk = (ei + ej) + ek;
-          const double denominator = epsabc - eijk;
-          // local offsets of the permutations inside their cubes
-          const int o_ijk = li + 8 * lj + CZ * lk, o_ikj = li + 8 * lk + CZ * lj,
-                    o_jik = lj + 8 * li + CZ * lk, o_jki = lj + 8 * lk + CZ * li,
-                    o_kij = lk + 8 * li + CZ * lj, o_kji = lk + 8 * lj + CZ * li;
-          // singles at a permutation (x,y,z): t_a[x] V^bc_yz + t_b[y] V^ac_xz
-          // + t_c[z] V^ab_xy; block slots: 0 -> i, 8 -> j, 16 -> k
-          // slot s = 8 * position + local index (position 0: i, 1: j, 2: k)
-          auto Z = [&](double wv, int sx, int sy, int sz, int, int, int) {
-            const int px = sx >> 3, py = sy >> 3, pz = sz >> 3;
-            const int lx = sx & 7, ly = sy & 7, lz = sz & 7;
-            double zz = wv + tA[sx] * sV[0][py][pz][ly + 8 * lz];
-            zz = zz + tB[sy] * sV[1][px][pz][lx + 8 * lz];
-            zz = zz + tC[sz] * sV[2][px][py][lx + 8 * ly];
-            return zz;
-          };
-          const int si = li, sj = 8 + lj, sk = 16 + lk;
-          if (distinct == 0) {
-            const double A = cube[alias[0]][o_ijk], B = cube[alias[1]][o_ikj],
-                         C = cube[alias[2]][o_jik], D = cube[alias[3]][o_jki],
-                         E = cube[alias[4]][o_kij], F = cube[alias[5]][o_kji];
-            const double U = Z(A, si, sj, sk, i, j, k), V = Z(B, si, sk, sj, i, k, j),
-                         W = Z(C, sj, si, sk, j, i, k), X = Z(D, sj, sk, si, j, k, i),
-                         Y = Z(E, sk, si, sj, k, i, j), Zz = Z(F, sk, sj, si, k, j, i);
-            const double UXY = U + (X + Y), VWZ = V + (W + Zz);
-            {
-              const double first =
-                  A * U + (B * V + (C * W + (D * X + (E * Y + F * Zz))));
-              const double second = (UXY - 2.0 * VWZ) * (A + (D + E));
-              const double third = (VWZ - 2.0 * UXY) * (B + (C + F));
-              const double value = 3.0 * first + (second + third);
-              e_acc += ((2.0 * value) / denominator) * (facjk * facij);
-            }
-            if constexpr (HJ) {
-              const double A2 = cubeJ[alias[0]][o_ijk], B2 = cubeJ[alias[1]][o_ikj],
-                           C2 = cubeJ[alias[2]][o_jik], D2 = cubeJ[alias[3]][o_jki],
-                           E2 = cubeJ[alias[4]][o_kij], F2 = cubeJ[alias[5]][o_kji];
-              const double first =
-                  A2 * U + (B2 * V + (C2 * W + (D2 * X + (E2 * Y + F2 * Zz))));
-              const double second = (UXY - 2.0 * VWZ) * (A2 + (D2 + E2));
-              const double third = (VWZ - 2.0 * UXY) * (B2 + (C2 + F2));
-              const double value = 3.0 * first + (second + third);
-              ect_acc += ((2.0 * value) / denominator) * (facjk * facij);
-            }
-          } else {
-            const double A = cube[alias[0]][o_ijk], B = cube[alias[3]][o_jki],
-                         C = cube[alias[4]][o_kij];
-            const double U = Z(A, si, sj, sk, i, j, k), V = Z(B, sj, sk, si, j, k, i),
-                         W = Z(C, sk, si, sj, k, i, j);
-            const double UVW = U + (V + W);
-            {
-              const double value =
-                  3.0 * ((A * U + B * V) + C * W) - (A + (B + C)) * UVW;
-              e_acc += ((2.0 * value) / denominator) * (facjk * facij);
-            }
-            if constexpr (HJ) {
-              const double A2 = cubeJ[alias[0]][o_ijk], B2 = cubeJ[alias[3]][o_jki],
-                           C2 = cubeJ[alias[4]][o_kij];
-              const double value =
-                  3.0 * ((A2 * U + B2 * V) + C2 * W) - (A2 + (B2 + C2)) * UVW;
-              ect_acc += ((2.0 * value) / denominator) * (facjk * facij);
-            }
-          }
-        }
-    }
+    triple_points<HJ, CZ, CSZ>(B1, B2, B3, no, distinct, epsabc, alias, cube,
+                               &cubeJ[0][0], tA, tB, tC, sE, sV, tid, e_acc,
+                               ect_acc);
     if (!more) break;
     B1 = N1;
     B2 = N2b;
@@ -409,6 +426,249 @@ void energy_kernel(EnergyParams P) {
     P.e_slot[2 * gslot] = e;
     P.e_slot[2 * gslot + 1] = have_j ? ect : e;
   }
+}
+
+// Fused class sums + singles + energy: one workgroup per (tuple slot, block
+// triple B1 >= B2 >= B3 of edge 8).  The workgroup gathers the (up to) six
+// permuted W cubes of the triple's orbit straight from the three class
+// buffers the doubles kernel wrote (each read along its own fastest index
+// in 64-byte row pieces) and sums them in LDS in the fixed order
+// (w + v) + u, the order of wsum_kernel, so W is the same bits as in the
+// cube path; then the triple's points follow triple_points, and the
+// workgroup's energy (a fixed shuffle/LDS tree) goes to its own entry of
+// e_part.  triple_sum_kernel adds a tuple's triples in order.  Nothing
+// about W leaves the chip: no cube is written to memory and read back.
+struct FusedParams {
+  const void *G, *GJ;  // class buffers [3][slots][No^3] (FP64 or FP32)
+  const double *Tai, *VHH, *ei, *ea;
+  const int32_t *windows;
+  const uint64_t *aprefix;
+  double *e_part;  // [slot][triple][2]
+  int64_t No, Nv, N2, W3g, slots, ntri;
+  uint64_t first, last;
+  int nb, nbk;
+  double tai_sign;
+};
+
+// Triple t of the nested order B3 outer, B2 = B3.., B1 = B2.. innermost
+// (the order the per-tuple kernel walks).
+__device__ __forceinline__ void triple_of(int t, int nbk, int &B1, int &B2,
+                                          int &B3) {
+  int r = t;
+  B3 = 0;
+  for (;;) {
+    const int m = nbk - B3, cnt = m * (m + 1) / 2;
+    if (r < cnt) break;
+    r -= cnt;
+    B3++;
+  }
+  B2 = B3;
+  for (;;) {
+    const int cnt = nbk - B2;
+    if (r < cnt) break;
+    r -= cnt;
+    B2++;
+  }
+  B1 = B2 + r;
+}
+
+template <typename GT, bool HJ>
+__global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
+  const int64_t gid = blockIdx.x;
+  const int64_t gslot = gid / P.ntri;
+  const int tri = (int)(gid % P.ntri);
+  const int nb = P.nb;
+  const int64_t nb3 = (int64_t)nb * nb * nb;
+  const int64_t w = gslot / nb3, s = gslot % nb3;
+  const int64_t ta = s / (nb * nb), tb = (s / nb) % nb, tc = s % nb;
+  const int64_t a = P.windows[3 * w] * (int64_t)nb + ta;
+  const int64_t b = P.windows[3 * w + 1] * (int64_t)nb + tb;
+  const int64_t c = P.windows[3 * w + 2] * (int64_t)nb + tc;
+  const int64_t No = P.No, Nv = P.Nv, N2 = P.N2;
+  const int tid = threadIdx.x;
+  bool ok = tuple_ok(Nv, a, b, c);
+  if (ok && (P.first > 0 || P.last != ~0ULL)) {
+    const uint64_t idx = tuple_index(P.aprefix, Nv, a, b, c);
+    ok = idx >= P.first && idx < P.last;
+  }
+  if (!ok) {
+    if (tid == 0) {
+      P.e_part[2 * gid] = 0.0;
+      P.e_part[2 * gid + 1] = 0.0;
+    }
+    return;
+  }
+  constexpr int CZ = 69, CSZ = 7 * CZ + 64;
+  __shared__ double cube[6][CSZ];
+  __shared__ double cubeJ[HJ ? 6 : 1][HJ ? CSZ : 1];
+  __shared__ double tA[24], tB[24], tC[24], sE[24];
+  __shared__ double sV[3][3][3][64];
+  __shared__ double red[2][4];
+
+  int distinct = 0;
+  if (a == b) distinct++;
+  if (b == c) distinct--;
+  const double epsabc = (P.ea[a] + P.ea[b]) + P.ea[c];
+  const int no = (int)No;
+  int B1, B2, B3;
+  triple_of(tri, P.nbk, B1, B2, B3);
+  // The orbit's distinct cubes.  Permutation p (ijk ikj jik jki kij kji)
+  // holds its W in the cube of block coordinates code[p] (i | j << 8 |
+  // k << 16); the distinct codes, in order of first appearance, are cubes
+  // 0 .. nc - 1 (qc), and alias[p] is the cube of permutation p.  Every
+  // array index below is a compile-time constant, so all of it stays in
+  // registers.
+  const int code[6] = {B1 | B2 << 8 | B3 << 16, B1 | B3 << 8 | B2 << 16,
+                       B2 | B1 << 8 | B3 << 16, B2 | B3 << 8 | B1 << 16,
+                       B3 | B1 << 8 | B2 << 16, B3 | B2 << 8 | B1 << 16};
+  int alias[6], qc[6], nc = 0;
+#pragma unroll
+  for (int q = 0; q < 6; q++) qc[q] = 0;
+#pragma unroll
+  for (int p = 0; p < 6; p++) {
+    int first = p;  // earliest permutation with the same cube
+#pragma unroll
+    for (int q = p - 1; q >= 0; q--)
+      if (code[q] == code[p]) first = q;
+    int al = nc;
+#pragma unroll
+    for (int q = 0; q < p; q++)
+      if (q == first) al = alias[q];
+    alias[p] = al;
+    if (first == p) {
+#pragma unroll
+      for (int q = 0; q < 6; q++)
+        if (q == nc) qc[q] = code[p];
+      nc++;
+    }
+  }
+
+  // element e = tid + 256 h of a cube, per class: row e >> 3 of the class's
+  // two slower indices, f = e & 7 along its fastest index
+  const GT *G = (const GT *)P.G;
+  const GT *GJ = (const GT *)P.GJ;
+  const int64_t W3g = P.W3g, slots = P.slots;
+  const int64_t bw = gslot * W3g, bv = (slots + gslot) * W3g,
+                bu = (2 * slots + gslot) * W3g;
+  // element (q, h) of class cls for this thread: its offset in the class
+  // buffer (or -1 outside the occupied range) and its LDS position
+  auto elem = [&](int q, int h, int cls, int &pos) -> int64_t {
+    const int e = tid + 256 * h, r = e >> 3, f = e & 7;
+    const int r0 = r & 7, r1 = r >> 3;
+    const int x = cls == 2 ? f : r0;
+    const int y = cls == 0 ? r1 : cls == 1 ? f : r0;
+    const int z = cls == 0 ? f : r1;
+    pos = x + 8 * y + CZ * z;
+    const int i = 8 * (qc[q] & 255) + x, j = 8 * ((qc[q] >> 8) & 255) + y,
+              k = 8 * (qc[q] >> 16) + z;
+    if (i >= no || j >= no || k >= no) return -1;
+    return cls == 0 ? bw + (int64_t)(i + no * j) * no + k      // w: k fastest
+         : cls == 1 ? bv + (int64_t)(i + no * k) * no + j      // v: j fastest
+                    : bu + (int64_t)(j + no * k) * no + i;     // u: i fastest
+  };
+  auto gather = [&](const GT *S, int q, int h, int cls) -> double {
+    int pos;
+    const int64_t off = elem(q, h, cls, pos);
+    const GT v = S[off >= 0 ? off : 0];  // always a valid address
+    return off >= 0 ? (double)v : 0.0;
+  };
+
+  // small tables of the triple: t at (a,x), (b,x), (c,x) and eps_i for x in
+  // the blocks B1, B2, B3; the V^bc, V^ac, V^ab block pairs
+  if (tid < 24) {
+    const int blk = tid >> 3 == 0 ? B1 : tid >> 3 == 1 ? B2 : B3;
+    const int x = blk * 8 + (tid & 7);
+    const bool in = x < no;
+    const int cx = in ? x : 0;
+    tA[tid] = in ? P.tai_sign * P.Tai[a + Nv * cx] : 0.0;
+    tB[tid] = in ? P.tai_sign * P.Tai[b + Nv * cx] : 0.0;
+    tC[tid] = in ? P.tai_sign * P.Tai[c + Nv * cx] : 0.0;
+    sE[tid] = P.ei[cx];
+  }
+  {
+    const double *Vbc = P.VHH + (b + Nv * c) * N2;
+    const double *Vac = P.VHH + (a + Nv * c) * N2;
+    const double *Vab = P.VHH + (a + Nv * b) * N2;
+    const int Bp[3] = {B1, B2, B3};
+#pragma unroll
+    for (int it = 0; it < 7; it++) {
+      const int e = tid + 256 * it;
+      const int v = e / 576, rs = (e / 64) % 9, l = e & 63;
+      const int r = rs / 3, sp = rs % 3;
+      const int y = Bp[r] * 8 + (l & 7), z = Bp[sp] * 8 + (l >> 3);
+      const double *V = v == 0 ? Vbc : v == 1 ? Vac : Vab;
+      const bool ok_ = e < 3 * 9 * 64 && r != sp && y < No && z < No;
+      const double val = V[ok_ ? y + No * z : 0];
+      if (e < 3 * 9 * 64) (&sV[0][0][0][0])[e] = ok_ ? val : 0.0;
+    }
+  }
+  // W (and the (cT) W) of the orbit's cubes: all loads first, then the
+  // three class phases into LDS in the order w, v, u
+  double vals[HJ ? 2 : 1][3][6][2];
+#pragma unroll
+  for (int set = 0; set < (HJ ? 2 : 1); set++)
+#pragma unroll
+    for (int cls = 0; cls < 3; cls++)
+#pragma unroll
+      for (int q = 0; q < 6; q++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+          if (q < nc) vals[set][cls][q][h] = gather(set ? GJ : G, q, h, cls);
+#pragma unroll
+  for (int cls = 0; cls < 3; cls++) {
+#pragma unroll
+    for (int q = 0; q < 6; q++)
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+        if (q < nc) {
+          int at;
+          (void)elem(q, h, cls, at);
+          if (cls == 0) {
+            cube[q][at] = vals[0][0][q][h];
+            if constexpr (HJ) cubeJ[q][at] = vals[HJ ? 1 : 0][0][q][h];
+          } else {
+            cube[q][at] += vals[0][cls][q][h];
+            if constexpr (HJ) cubeJ[q][at] += vals[HJ ? 1 : 0][cls][q][h];
+          }
+        }
+    __syncthreads();
+  }
+
+  double e_acc = 0.0, ect_acc = 0.0;
+  triple_points<HJ, CZ, CSZ>(B1, B2, B3, no, distinct, epsabc, alias, cube,
+                             &cubeJ[0][0], tA, tB, tC, sE, sV, tid, e_acc,
+                             ect_acc);
+  e_acc = wave_sum(e_acc);
+  ect_acc = wave_sum(ect_acc);
+  const int wave = tid >> 6, lane = tid & 63;
+  if (lane == 0) {
+    red[0][wave] = e_acc;
+    red[1][wave] = ect_acc;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const double e = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    const double ect = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    P.e_part[2 * gid] = e;
+    P.e_part[2 * gid + 1] = HJ ? ect : e;
+  }
+}
+
+// Energy of each tuple slot: its triples' partials added in triple order.
+__global__ __launch_bounds__(256) void triple_sum_kernel(const double *e_part,
+                                                         int64_t slots,
+                                                         int64_t ntri,
+                                                         double *e_slot) {
+  const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (slot >= slots) return;
+  const double *p = e_part + 2 * slot * ntri;
+  double e = 0.0, ect = 0.0;
+  for (int64_t t = 0; t < ntri; t++) {
+    e += p[2 * t];
+    ect += p[2 * t + 1];
+  }
+  e_slot[2 * slot] = e;
+  e_slot[2 * slot + 1] = ect;
 }
 
 // Compensated sum of the slot energies of one batch: 256 threads each sum a
@@ -474,10 +734,54 @@ hipError_t launch_wsum(const DevSystem &s, const BatchArgs &b, const void *A,
 
 hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
                          const void *G, const void *GJ, bool g32,
-                         double tai_sign, double *e_slot, double *wcube,
-                         hipStream_t st) {
+                         double tai_sign, double *e_slot, double *scratch,
+                         bool fused, hipStream_t st) {
+  if (fused) {
+    const int64_t nbk = w_blocks(s.No), ntri = n_triples(nbk);
+    FusedParams P;
+    P.G = G;
+    P.GJ = GJ;
+    P.Tai = s.Tai;
+    P.VHH = s.VHH;
+    P.ei = s.ei;
+    P.ea = s.ea;
+    P.windows = b.windows;
+    P.aprefix = s.aprefix;
+    P.e_part = scratch;
+    P.No = s.No;
+    P.Nv = s.Nv;
+    P.N2 = s.No * s.No;
+    P.W3g = s.No * s.No * s.No;
+    P.slots = b.slots;
+    P.ntri = ntri;
+    P.first = b.first;
+    P.last = b.last;
+    P.nb = b.nb;
+    P.nbk = (int)nbk;
+    P.tai_sign = tai_sign;
+    const int64_t grid = b.slots * ntri;
+    if (grid <= 0) return hipSuccess;
+    if (g32 && GJ)
+      hipLaunchKernelGGL((energy_fused_kernel<float, true>), dim3((unsigned)grid),
+                         dim3(256), 0, st, P);
+    else if (g32)
+      hipLaunchKernelGGL((energy_fused_kernel<float, false>), dim3((unsigned)grid),
+                         dim3(256), 0, st, P);
+    else if (GJ)
+      hipLaunchKernelGGL((energy_fused_kernel<double, true>), dim3((unsigned)grid),
+                         dim3(256), 0, st, P);
+    else
+      hipLaunchKernelGGL((energy_fused_kernel<double, false>), dim3((unsigned)grid),
+                         dim3(256), 0, st, P);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(triple_sum_kernel, dim3((unsigned)((b.slots + 255) / 256)),
+                       dim3(256), 0, st, (const double *)scratch, b.slots, ntri,
+                       e_slot);
+    return hipGetLastError();
+  }
   const int64_t W3c = w_cube(s.No);
-  double *W = wcube, *WJ = GJ ? wcube + b.slots * W3c : nullptr;
+  double *W = scratch, *WJ = GJ ? scratch + b.slots * W3c : nullptr;
   hipError_t e = launch_wsum(s, b, G, nullptr, g32, W, st);
   if (e != hipSuccess) return e;
   if (GJ && (e = launch_wsum(s, b, GJ, nullptr, g32, WJ, st)) != hipSuccess)

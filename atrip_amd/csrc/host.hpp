@@ -7,6 +7,7 @@
 
 #include <exception>
 #include <string>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -66,6 +67,19 @@ uint64_t system_bytes(int64_t No, int64_t Nv, bool with_j, int store);
 void run_fixed(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
                uint64_t first, uint64_t count, void *base, size_t bytes,
                double *energy, double *ct_energy);
+
+// Windows of a system's plan at the context's window edge (lexicographic
+// block triples holding a tuple).
+uint64_t plan_windows(atrip_hip_ctx *ctx, int sys);
+// Plain-path run (doubles kernel, energy epilogue) over plan windows
+// [wbegin, wend) of a system, every tuple of each, in launches of the
+// context's "batch" windows.  out receives per window, in plan order, the
+// (sum, comp) of a compensated sum of its raw tuple energies in slot order,
+// so a window's pair does not depend on how windows were grouped into
+// launches or calls.
+void run_plan_windows(atrip_hip_ctx *ctx, int sys, int precision,
+                      uint64_t wbegin, uint64_t wend,
+                      std::vector<double> &out);
 
 // Number of tuples a <= b <= c < Nv, not all equal.
 inline uint64_t n_tuples(uint64_t Nv) {

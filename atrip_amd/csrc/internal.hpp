@@ -69,6 +69,10 @@ __host__ __device__ inline int64_t w_cube(int64_t No) {
   const int64_t n = w_blocks(No) * WCUBE;
   return n * n * n;
 }
+// Block triples B1 >= B2 >= B3 over nbk blocks per index.
+__host__ __device__ inline int64_t n_triples(int64_t nbk) {
+  return nbk * (nbk + 1) * (nbk + 2) / 6;
+}
 
 // Device-resident system.  Pair-indexed arrays use the CTF pair order
 // p(x,y) = x + Nv*y.  Layouts (all row-major with the last index fastest):
@@ -203,6 +207,8 @@ struct atrip_hip_ctx {
   int timing = 0;
   int tile = -1;  // -1: chosen per system shape
   int operand_store = 0;  // store mode of the next uploads (DevSystem::store)
+  int epilogue = 0;  // 0: fused class sums + energy, 1: W cubes first
+  int serve_plain_no = 16;  // served requests with No >= this: plain path
   int64_t stage_kb = 4 << 20;  // device staging of host uploads (KiB)
   // scratch, grown on demand
   double *G = nullptr;       // class buffers [ncls][slots][No^3]
@@ -265,11 +271,20 @@ hipError_t launch_wsum(const DevSystem &s, const BatchArgs &b, const void *A,
                        const void *B, bool g32, double *out, hipStream_t st);
 
 // Class sums and singles + energy epilogue; writes e_slot[2*slot + {0,1}].
-// wcube: scratch of (GJ ? 2 : 1) * slots * w_cube(No) doubles.
+// fused: one kernel gathers each block triple's W straight from the class
+// buffers (scratch: slots * n_triples * 2 doubles of per-triple partials);
+// otherwise the class-sum kernel writes W as cubes first (scratch: (GJ ? 2
+// : 1) * slots * w_cube(No) doubles).  Same W bits either way.
 hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
                          const void *G, const void *GJ, bool g32,
-                         double tai_sign, double *e_slot, double *wcube,
-                         hipStream_t st);
+                         double tai_sign, double *e_slot, double *scratch,
+                         bool fused, hipStream_t st);
+// Doubles of energy scratch per tuple slot (see launch_energy).
+inline int64_t energy_scratch_per_slot(int64_t No, bool use_j, bool cplx,
+                                       bool fused) {
+  if (fused && !cplx) return 2 * n_triples(w_blocks(No));
+  return (use_j ? 2 : 1) * (cplx ? 2 : 1) * w_cube(No);
+}
 
 // Singles + energy of complex systems from the four class-buffer sets of
 // the real passes (Wr = G0 + G1, Wi = G2 + G3); GJ null without (cT).

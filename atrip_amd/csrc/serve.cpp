@@ -8,14 +8,19 @@
 //      until its energy is done);
 //   2. picks windows from the active requests (shortest remaining work
 //      first, or an equal share each) up to a target iteration time, and
-//      runs them all in one persistent launch of the batched kernel, so the
-//      work of requests admitted at different times shares the device;
+//      runs the windows of small requests in one persistent launch of the
+//      batched kernel, and those of large requests (No at or above the
+//      context's "serve_plain_no", and every bf16x3 request) through the
+//      plain path's class-GEMM launches, so the work of requests admitted at
+//      different times shares the device in every step;
 //   3. adds each request's window energies to its running compensated sum
 //      in canonical window order and retires the requests whose windows are
 //      all done, freeing their memory for the queue.
-// A request's energy is the same bits alone or in any mix: every window is
-// computed by one workgroup independently of the others, and its sum runs
-// over the same window sequence however the windows were split over steps.
+// A request's energy is the same bits alone or in any mix: every window's
+// (sum, comp) pair is computed independently of the other windows (by one
+// workgroup of the batched kernel, or from the window's tuple energies in
+// slot order on the plain path), and the request's sum runs over the same
+// window sequence however the windows were split over steps.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,8 +52,10 @@ struct Request {
   int sys = 0;
   uint64_t bytes = 0;
   BatchTune plan;
-  std::vector<Win> wins;
-  size_t cursor = 0;
+  std::vector<Win> wins;     // batched windows
+  bool plain = false;        // windows run on the plain path
+  size_t nwin = 0;           // windows of the request (either kind)
+  size_t cursor = 0;         // windows done
   double sum = 0.0, comp = 0.0, energy = 0.0;
   uint64_t tuples = 0;
   double t_submit = 0, t_admit = 0, t_done = 0;  // ms on the server clock
@@ -96,20 +103,39 @@ void retire(atrip_hip_server *srv, Request &r) {
   srv->resident -= r.bytes;
 }
 
+// Whether a request's windows run on the plain path: large requests (the
+// plain class-GEMM kernels reach a far higher fraction of the MFMA peak once
+// No^2 rows fill their tiles), No beyond the batched kernel's limit, and
+// every bf16x3 request (a plain-path form only).
+bool runs_plain(const atrip_hip_server *srv, int64_t No) {
+  return srv->precision == ATRIP_BF16X3 || No > MAX_NO ||
+         No >= srv->ctx->serve_plain_no;
+}
+
+// Initial estimate of the workgroup-seconds of one plain-path window (every
+// workgroup of the grid busy for its GPU time); replaced by the measured
+// figure after the request's first step.
+double plain_window_estimate(int64_t No, int64_t Nv, int nb, int precision,
+                             int grid) {
+  const double tflops = precision == ATRIP_FP64         ? 50.0
+                        : precision == ATRIP_SPLIT_FP32 ? 70.0
+                        : precision == ATRIP_FP32       ? 85.0
+                                                        : 120.0;
+  const double flop = 12.0 * (double)No * No * No * (double)(No + Nv) *
+                      (double)nb * nb * nb;
+  return flop / (tflops * 1e12) * grid;
+}
+
 // Admit queued requests in arrival order while they fit.
 void admit(atrip_hip_server *srv, int grid) {
   atrip_hip_ctx *ctx = srv->ctx;
   while (!srv->queue.empty() && (int)srv->active.size() < srv->max_active) {
     Request &r = srv->reqs.at(srv->queue.front());
-    if (r.No > MAX_NO) {
-      fail_request(srv, r, "No > 64 is not served (use atrip_hip_run)");
-      srv->queue.pop_front();
-      continue;
-    }
     // served systems keep only the operand form the server computes in
     const int store = srv->precision == ATRIP_FP64         ? 0
                       : srv->precision == ATRIP_SPLIT_FP32 ? 1
-                                                           : 2;
+                      : srv->precision == ATRIP_FP32       ? 2
+                                                           : 3;
     r.bytes = system_bytes(r.No, r.Nv, false, store);
     if (r.bytes > srv->budget) {
       fail_request(srv, r, "system larger than the HBM budget");
@@ -140,10 +166,20 @@ void admit(atrip_hip_server *srv, int grid) {
     r.t_admit = srv->now();
     srv->resident += r.bytes;
     srv->peak_resident = std::max(srv->peak_resident, srv->resident);
-    r.plan = plan_for(ctx, get_sys(ctx, sys), grid, srv->precision);
-    r.wins = windows_of(r.Nv, r.plan.nb);
+    r.plain = runs_plain(srv, r.No);
+    if (r.plain) {
+      r.nwin = plan_windows(ctx, sys);
+      r.plan.nb = ctx->window;
+      r.plan.measured = false;
+      r.plan.sec_window = plain_window_estimate(r.No, r.Nv, ctx->window,
+                                                srv->precision, grid);
+    } else {
+      r.plan = plan_for(ctx, get_sys(ctx, sys), grid, srv->precision);
+      r.wins = windows_of(r.Nv, r.plan.nb);
+      r.nwin = r.wins.size();
+    }
     r.cursor = 0;
-    if (r.wins.empty())
+    if (r.nwin == 0)
       retire(srv, r);
     else
       srv->active.push_back(r.id);
@@ -168,7 +204,7 @@ int step(atrip_hip_server *srv) {
   std::vector<int64_t> order_req = srv->active;
   auto remaining = [&](int64_t id) {
     const Request &r = srv->reqs.at(id);
-    return (double)(r.wins.size() - r.cursor) * r.plan.sec_window;
+    return (double)(r.nwin - r.cursor) * r.plan.sec_window;
   };
   if (srv->policy == 0)
     std::stable_sort(order_req.begin(), order_req.end(),
@@ -179,15 +215,22 @@ int step(atrip_hip_server *srv) {
   std::vector<BSplit> split;
   std::vector<BTask> tasks;
   std::vector<double> cost;
-  std::vector<std::pair<int64_t, size_t>> took;  // request, windows taken
+  std::vector<std::pair<int64_t, size_t>> took;   // batched: request, windows
+  std::vector<std::pair<int64_t, size_t>> plain;  // plain path: request, windows
   const double share = target / (double)order_req.size();
   for (int64_t id : order_req) {
-    if (srv->policy == 0 && left <= 0 && !tasks.empty()) break;
+    if (srv->policy == 0 && left <= 0 && (!tasks.empty() || !plain.empty()))
+      break;
     Request &r = srv->reqs.at(id);
     const double sw = std::max(r.plan.sec_window, 1e-9);
     const double want = srv->policy == 0 ? left : share;
     size_t k = (size_t)std::ceil(std::max(want, 0.0) / sw);
-    k = std::max<size_t>(1, std::min(k, r.wins.size() - r.cursor));
+    k = std::max<size_t>(1, std::min(k, r.nwin - r.cursor));
+    left -= (double)k * sw;
+    if (r.plain) {
+      plain.push_back({id, k});
+      continue;
+    }
     const int si = (int)sys.size();
     DevSystem &ds = get_sys(ctx, r.sys);
     sys.push_back(make_bsys(ctx, ds, r.plan, srv->precision));
@@ -199,30 +242,51 @@ int step(atrip_hip_server *srv) {
       cost.push_back(r.plan.sec_window * (0.5 + 0.5 * w.valid / nb3));
     }
     took.push_back({id, k});
-    left -= (double)k * sw;
   }
-  std::vector<int32_t> order(tasks.size());
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
-  const LaunchOut out =
-      run_tasks(ctx, srv->precision, sys, split, tasks, order, grid, true);
-  srv->busy_ms += out.ms;
-  srv->launches++;
-
-  // accumulate in canonical window order and retire
-  size_t t = 0;
-  for (auto &tk : took) {
-    Request &r = srv->reqs.at(tk.first);
-    for (size_t q = 0; q < tk.second; q++, t++) {
-      neumaier_add(r.sum, r.comp, out.task_e[2 * t], out.task_e[2 * t + 1]);
-      r.tuples += (uint64_t)r.wins[r.cursor + q].valid;
+  if (!tasks.empty()) {
+    std::vector<int32_t> order(tasks.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
+    const LaunchOut out =
+        run_tasks(ctx, srv->precision, sys, split, tasks, order, grid, true);
+    srv->busy_ms += out.ms;
+    srv->launches++;
+    // accumulate in canonical window order
+    size_t t = 0;
+    for (auto &tk : took) {
+      Request &r = srv->reqs.at(tk.first);
+      for (size_t q = 0; q < tk.second; q++, t++) {
+        neumaier_add(r.sum, r.comp, out.task_e[2 * t], out.task_e[2 * t + 1]);
+        r.tuples += (uint64_t)r.wins[r.cursor + q].valid;
+      }
+      r.cursor += tk.second;
     }
-    r.cursor += tk.second;
+  }
+  // the plain-path windows, request by request, in plan order
+  std::vector<double> wsum;
+  for (auto &pk : plain) {
+    Request &r = srv->reqs.at(pk.first);
+    const Clock::time_point t0 = Clock::now();
+    run_plan_windows(ctx, r.sys, srv->precision, r.cursor, r.cursor + pk.second,
+                     wsum);
+    const double ms =
+        std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    if (wsum.size() != 2 * pk.second) throw Fail{"plain-path window count"};
+    for (size_t q = 0; q < pk.second; q++)
+      neumaier_add(r.sum, r.comp, wsum[2 * q], wsum[2 * q + 1]);
+    r.cursor += pk.second;
+    srv->busy_ms += ms;
+    srv->launches++;
+    // measured workgroup-seconds per window from now on
+    r.plan.sec_window = ms * 1e-3 / (double)pk.second * grid;
+    r.plan.measured = true;
+    took.push_back(pk);
   }
   for (auto &tk : took) {
     Request &r = srv->reqs.at(tk.first);
-    if (r.cursor < r.wins.size()) continue;
+    if (r.cursor < r.nwin) continue;
+    if (r.plain) r.tuples = n_tuples((uint64_t)r.Nv);
     srv->tuples_done += r.tuples;
     retire(srv, r);
     srv->active.erase(std::find(srv->active.begin(), srv->active.end(), r.id));
@@ -250,7 +314,9 @@ int atrip_hip_serve_create(atrip_hip_ctx *ctx, uint64_t hbm_budget_bytes,
                            atrip_hip_server **out) {
   return guarded(ctx, [&] {
     if (!out) throw Fail{"null argument"};
-    check_batched_precision(precision);
+    if (precision != ATRIP_FP64 && precision != ATRIP_SPLIT_FP32 &&
+        precision != ATRIP_FP32 && precision != ATRIP_BF16X3)
+      throw Fail{"unknown precision " + std::to_string(precision)};
     if (hbm_budget_bytes == 0) throw Fail{"HBM budget must be positive"};
     if (max_active < 1) throw Fail{"max_active must be positive"};
     if (!(iter_ms > 0)) throw Fail{"iter_ms must be positive"};

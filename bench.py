@@ -192,11 +192,15 @@ def cpu_baseline(no: int, nv: int, seed: int, budget_s: float):
     }
 
 
-def bench_batched(ctx, n_systems: int, seed: int):
-    """Batched form: n seeded small systems, autotuned, one persistent launch."""
+def bench_batched(ctx, n_systems: int, seed: int, shape=None):
+    """Batched form: n seeded small systems, autotuned, one persistent launch
+    (the seeded workload's shapes, or n systems of one (No, Nv) shape)."""
     from atrip_amd import workloads as W
 
-    shapes = W.batch_shapes(n_systems, seed=seed)
+    if shape:
+        shapes = [(shape[0], shape[1], 1000 + s) for s in range(n_systems)]
+    else:
+        shapes = W.batch_shapes(n_systems, seed=seed)
     t0 = time.time()
     systems = [ctx.upload_synthetic(no, nv, s) for no, nv, s in shapes]
     up_s = time.time() - t0
@@ -217,8 +221,10 @@ def bench_batched(ctx, n_systems: int, seed: int):
     for s_ in systems:
         ctx.release(s_)
     return {
-        "workload": (f"{n_systems} systems, No~U{{4..16}}, Nv~U{{20..120}}, "
-                     f"shape seed {seed}, tensor seeds 1000+s; FP64"),
+        "workload": ((f"{n_systems} systems, No~U{{4..16}}, Nv~U{{20..120}}, "
+                      f"shape seed {seed}, tensor seeds 1000+s; FP64") if not shape else
+                     (f"{n_systems} systems of No={shape[0]} Nv={shape[1]}, "
+                      f"tensor seeds 1000+s; FP64")),
         "tuples": tuples,
         "value": tuples / wall, "unit": "tuples/s",
         "tflops": flop / wall / 1e12,
@@ -231,7 +237,7 @@ def bench_batched(ctx, n_systems: int, seed: int):
 
 
 def bench_serving(ctx, n_requests: int, rate: float, seed: int,
-                  precisions=("fp64", "split_fp32")):
+                  precisions=("fp64", "bf16x3", "split_fp32")):
     """The same Poisson request stream served once per precision; the mixed
     runs also report each request's relative energy difference to FP64."""
     from atrip_amd import serve
@@ -246,7 +252,9 @@ def bench_serving(ctx, n_requests: int, rate: float, seed: int,
         e = r.pop("energies")
         r["workload"] = (f"{n_requests} requests, Poisson {rate} req/s, No log-U[8,40], "
                          f"Nv log-U[40,320], seed {seed}, tensor seeds 5000+r; "
-                         f"HBM budget 200 GB; {prec}")
+                         f"HBM budget 200 GB; {prec}; requests with No >= "
+                         f"{ctx.get_option('serve_plain_no')} on the plain path, "
+                         f"the rest in the persistent batched launch")
         if ref is None:
             ref = e
         else:
@@ -359,6 +367,10 @@ def main(argv=None):
     ap.add_argument("--window", type=int, default=8)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--tile", type=int, default=-1)
+    ap.add_argument("--operand-store", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="operand store of the timed system: 0 FP64 (mixed "
+                         "copies made on first use), 1 split FP32, 2 FP32, "
+                         "3 bf16x3 only (No=60 Nv=600 mixed runs)")
     ap.add_argument("--slices", type=int, default=16,
                     help="steps that cover the tuple space once")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
@@ -371,6 +383,9 @@ def main(argv=None):
                          "(split_fp32, fp32, bf16x3)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=256)
+    ap.add_argument("--batch-shape", default=None,
+                    help="No,Nv: batched systems all of this shape instead of "
+                         "the seeded workload's")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--no-c3", action="store_true",
@@ -398,7 +413,11 @@ def main(argv=None):
     ctx.set_option("batch", args.batch)
     ctx.set_option("tile", args.tile)
     t0 = time.time()
-    sys_ = ctx.upload_synthetic(args.no, args.nv, args.seed)
+    ctx.set_option("operand_store", args.operand_store)
+    try:
+        sys_ = ctx.upload_synthetic(args.no, args.nv, args.seed)
+    finally:
+        ctx.set_option("operand_store", 0)
     gen_s = time.time() - t0
     slices = step_slices(args.nv, args.window, args.slices)
 
@@ -436,7 +455,7 @@ def main(argv=None):
     value = done / elapsed
 
     mixed = None
-    if args.precision == "fp64" and not args.no_mixed:
+    if args.precision == "fp64" and args.operand_store == 0 and not args.no_mixed:
         # the same steps in each mixed form on the same resident inputs
         mixed = {}
         for form in [f for f in args.mixed.split(",") if f]:
@@ -472,7 +491,9 @@ def main(argv=None):
 
     ctx.release(sys_)
     log("batched form")
-    batched = None if args.no_batched else bench_batched(ctx, args.batch_systems, 0)
+    bshape = tuple(int(x) for x in args.batch_shape.split(",")) if args.batch_shape else None
+    batched = (None if args.no_batched else
+               bench_batched(ctx, args.batch_systems, 0, bshape))
     log("serving form")
     serving = None if args.no_serve else bench_serving(ctx, args.serve_requests, 20.0, 0)
     c3 = None
@@ -508,7 +529,8 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32x2(split)+f64",
+        "dtype": {"fp64": "f64", "split_fp32": "f32x2(split)+f64",
+                  "fp32": "f32+f64", "bf16x3": "bf16x3+f32acc+f64"}[args.precision],
         "data": "synthetic (seeded counter-based generator, generated on device)",
         "config": {
             "workload": (f"plain (T) No={args.no} Nv={args.nv}, all tuples "

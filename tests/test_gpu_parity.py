@@ -433,3 +433,35 @@ def test_complex_with_zero_imaginary_is_real(hip_ctx, oracle):
         hip_ctx.release(s_r)
         hip_ctx.release(s_z)
     assert abs(ez - er) <= 1e-13 * abs(er) and abs(ectz - ectr) <= 1e-13 * abs(ectr)
+
+
+
+def test_fused_and_cube_epilogues_agree(hip_ctx):
+    """The fused epilogue (W gathered per block triple from the class
+    buffers, energies summed per triple) and the cube epilogue (W written
+    as cubes first, one workgroup per tuple) form the same W bits and
+    differ only in the order the points' terms are added: per-tuple
+    energies within 1e-14 of the largest, E(T) and E(cT) within 1e-14
+    relative, for FP64, bf16x3 and split FP32 with (cT)."""
+    hip_ctx.set_option("window", 4)
+    hip_ctx.set_option("batch", 3)
+    try:
+        s_ = hip_ctx.upload_synthetic(11, 37, 21, with_j=True)
+        n = 37 * 38 * 39 // 6 - 37
+        for prec in ("fp64", "bf16x3", "split_fp32"):
+            print("epilogues", prec, flush=True)
+            out = {}
+            for epi in (0, 1):
+                hip_ctx.set_option("epilogue", epi)
+                out[epi] = (hip_ctx.run(s_, prec),
+                            hip_ctx.tuple_energies(s_, 0, n, precision=prec, with_ct=True))
+            (e0, c0), (t0, u0) = out[0]
+            (e1, c1), (t1, u1) = out[1]
+            assert abs(e0 - e1) <= 1e-14 * abs(e1) and abs(c0 - c1) <= 1e-14 * abs(c1)
+            assert np.max(np.abs(t0 - t1)) <= 1e-14 * np.max(np.abs(t1))
+            assert np.max(np.abs(u0 - u1)) <= 1e-14 * np.max(np.abs(u1))
+        hip_ctx.release(s_)
+    finally:
+        hip_ctx.set_option("epilogue", 0)
+        hip_ctx.set_option("window", 8)
+        hip_ctx.set_option("batch", 4)

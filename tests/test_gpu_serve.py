@@ -173,3 +173,62 @@ def test_mixed_precision_serving(hip_ctx):
         # the plain path's split kernel accumulates on 32x32 MFMAs, the
         # batched kernel behind the server on 16x16 ones
         assert _close(alone[m], ep, 1e-7)
+
+
+@pytest.mark.parametrize("plain_no", [1, 12, 1000])
+def test_plain_path_routing(hip_ctx, alone, plain_no):
+    """Requests with No >= serve_plain_no run their windows through the plain
+    class-GEMM launches, the rest through the persistent batched launch;
+    whatever the split, every result is bitwise the same alone and in the
+    mix, and within 1e-12 of the plain path's atrip_hip_run."""
+    print("serve plain routing, serve_plain_no", plain_no, flush=True)
+    hip_ctx.set_option("serve_plain_no", plain_no)
+    try:
+        solo = {}
+        for m in MIX:
+            with serve.Server(hip_ctx, hbm_budget_gb=50) as srv:
+                rid = srv.submit_synthetic(*m)
+                solo[m] = srv.drain([rid])[rid]["energy"]
+        with serve.Server(hip_ctx, hbm_budget_gb=50, iter_ms=5.0) as srv:
+            ids = [srv.submit_synthetic(*MIX[q]) for q in range(2)]
+            for q in range(2, len(MIX)):
+                srv.step()
+                ids.append(srv.submit_synthetic(*MIX[q]))
+            res = srv.drain(ids)
+    finally:
+        hip_ctx.set_option("serve_plain_no", 16)
+    for rid, m in zip(ids, MIX):
+        assert res[rid]["state"] == serve.DONE
+        assert res[rid]["energy"] == solo[m], m
+        assert _close(solo[m], alone[m[2]][1], REL_PLAIN), m
+
+
+def test_bf16x3_serving(hip_ctx):
+    """A bf16x3 server (every request on the plain path, bf16x3 operands
+    only): each result bitwise the same alone and in the mix, equal to the
+    plain path's bf16x3 run to rounding, and within the mixed tolerance of
+    FP64."""
+    print("serve bf16x3", flush=True)
+    mix = MIX[:5]
+    solo = {}
+    for m in mix:
+        with serve.Server(hip_ctx, hbm_budget_gb=50, precision="bf16x3") as srv:
+            rid = srv.submit_synthetic(*m)
+            solo[m] = srv.drain([rid])[rid]["energy"]
+    with serve.Server(hip_ctx, hbm_budget_gb=50, iter_ms=2.0,
+                      precision="bf16x3") as srv:
+        ids = [srv.submit_synthetic(*mix[0])]
+        for m in mix[1:]:
+            srv.step()
+            ids.append(srv.submit_synthetic(*m))
+        res = srv.drain(ids)
+    for rid, m in zip(ids, mix):
+        assert res[rid]["energy"] == solo[m], m
+        s = hip_ctx.upload_synthetic(*m)
+        try:
+            e3, _ = hip_ctx.run(s, "bf16x3")
+            e64, _ = hip_ctx.run(s)
+        finally:
+            hip_ctx.release(s)
+        assert _close(solo[m], e3, 1e-12), m
+        assert _close(solo[m], e64, 1e-7), m
