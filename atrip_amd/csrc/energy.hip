@@ -162,12 +162,19 @@ __device__ __forceinline__ void orbit(int B1, int B2, int B3, int nbk,
 // sV.  Z = W + singles (reference src/atrip/Equations.cxx:387-426) and the
 // get_energy_distinct / get_energy_same formulas (:101-238) in the
 // reference's expression order.
-template <bool HJ, int CZ, int CSZ>
+// sV holds the 8x8 V blocks of the triple, [slice][block pair][ly + 8 lz]:
+// all nine ordered position pairs (V6 false) or only the six with two
+// different positions (V6 true), the only ones a point reads.
+template <bool HJ, int CZ, int CSZ, bool V6>
 __device__ __forceinline__ void triple_points(
     int B1, int B2, int B3, int no, int distinct, double epsabc,
     const int (&alias)[6], const double (*cube)[CSZ], const double *cubeJ,
     const double *tA, const double *tB, const double *tC, const double *sE,
-    const double (*sV)[3][3][64], int tid, double &e_acc, double &ect_acc) {
+    const double *sV, int tid, double &e_acc, double &ect_acc) {
+  auto vb = [](int v, int r, int s2, int q) {
+    return V6 ? (v * 6 + 2 * r + s2 - (s2 > r ? 1 : 0)) * 64 + q
+              : ((v * 3 + r) * 3 + s2) * 64 + q;
+  };
   // both points of a thread in one body, so their LDS reads and FP64
   // chains interleave
 #pragma unroll
@@ -190,9 +197,9 @@ __device__ __forceinline__ void triple_points(
     auto Z = [&](double wv, int sx, int sy, int sz) {
       const int px = sx >> 3, py = sy >> 3, pz = sz >> 3;
       const int lx = sx & 7, ly = sy & 7, lz = sz & 7;
-      double zz = wv + tA[sx] * sV[0][py][pz][ly + 8 * lz];
-      zz = zz + tB[sy] * sV[1][px][pz][lx + 8 * lz];
-      zz = zz + tC[sz] * sV[2][px][py][lx + 8 * ly];
+      double zz = wv + tA[sx] * sV[vb(0, py, pz, ly + 8 * lz)];
+      zz = zz + tB[sy] * sV[vb(1, px, pz, lx + 8 * lz)];
+      zz = zz + tC[sz] * sV[vb(2, px, py, lx + 8 * ly)];
       return zz;
     };
     auto J = [&](int p, int o) { return cubeJ[alias[p] * CSZ + o]; };
@@ -400,9 +407,10 @@ void energy_kernel(EnergyParams P) {
       int cb_[6], nc_;
       orbit(B1, B2, B3, nbk, cb_, nc_, alias);
     }
-    triple_points<HJ, CZ, CSZ>(B1, B2, B3, no, distinct, epsabc, alias, cube,
-                               &cubeJ[0][0], tA, tB, tC, sE, sV, tid, e_acc,
-                               ect_acc);
+    triple_points<HJ, CZ, CSZ, false>(B1, B2, B3, no, distinct, epsabc,
+                                      alias, cube, &cubeJ[0][0], tA, tB, tC,
+                                      sE, &sV[0][0][0][0], tid, e_acc,
+                                      ect_acc);
     if (!more) break;
     B1 = N1;
     B2 = N2b;
@@ -501,8 +509,11 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
   constexpr int CZ = 69, CSZ = 7 * CZ + 64;
   __shared__ double cube[6][CSZ];
   __shared__ double cubeJ[HJ ? 6 : 1][HJ ? CSZ : 1];
-  __shared__ double tA[24], tB[24], tC[24], sE[24];
-  __shared__ double sV[3][3][3][64];
+  // singles tables (entries 0..23 used; wave 0 writes all 64) and the V
+  // blocks of the six block pairs of different positions (1152 used; every
+  // thread writes its five, 1280)
+  __shared__ double tA[64], tB[64], tC[64], sE[64];
+  __shared__ double sV[5 * 256];
   __shared__ double red[2][4];
 
   int distinct = 0;
@@ -542,6 +553,12 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
       nc++;
     }
   }
+  // slots past the distinct cubes repeat cube 0: their loads are issued
+  // like the others (none waits behind a branch) and hit in cache; what they
+  // write to LDS is never read
+#pragma unroll
+  for (int q = 1; q < 6; q++)
+    if (q >= nc) qc[q] = qc[0];
 
   // element e = tid + 256 h of a cube, per class: row e >> 3 of the class's
   // two slower indices, f = e & 7 along its fastest index
@@ -573,37 +590,11 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
     return off >= 0 ? (double)v : 0.0;
   };
 
-  // small tables of the triple: t at (a,x), (b,x), (c,x) and eps_i for x in
-  // the blocks B1, B2, B3; the V^bc, V^ac, V^ab block pairs
-  if (tid < 24) {
-    const int blk = tid >> 3 == 0 ? B1 : tid >> 3 == 1 ? B2 : B3;
-    const int x = blk * 8 + (tid & 7);
-    const bool in = x < no;
-    const int cx = in ? x : 0;
-    tA[tid] = in ? P.tai_sign * P.Tai[a + Nv * cx] : 0.0;
-    tB[tid] = in ? P.tai_sign * P.Tai[b + Nv * cx] : 0.0;
-    tC[tid] = in ? P.tai_sign * P.Tai[c + Nv * cx] : 0.0;
-    sE[tid] = P.ei[cx];
-  }
-  {
-    const double *Vbc = P.VHH + (b + Nv * c) * N2;
-    const double *Vac = P.VHH + (a + Nv * c) * N2;
-    const double *Vab = P.VHH + (a + Nv * b) * N2;
-    const int Bp[3] = {B1, B2, B3};
-#pragma unroll
-    for (int it = 0; it < 7; it++) {
-      const int e = tid + 256 * it;
-      const int v = e / 576, rs = (e / 64) % 9, l = e & 63;
-      const int r = rs / 3, sp = rs % 3;
-      const int y = Bp[r] * 8 + (l & 7), z = Bp[sp] * 8 + (l >> 3);
-      const double *V = v == 0 ? Vbc : v == 1 ? Vac : Vab;
-      const bool ok_ = e < 3 * 9 * 64 && r != sp && y < No && z < No;
-      const double val = V[ok_ ? y + No * z : 0];
-      if (e < 3 * 9 * 64) (&sV[0][0][0][0])[e] = ok_ ? val : 0.0;
-    }
-  }
-  // W (and the (cT) W) of the orbit's cubes: all loads first, then the
-  // three class phases into LDS in the order w, v, u
+  // Every global load of the workgroup is issued before any is waited on
+  // (a load behind a branch is waited on before the branch joins, which
+  // serialised the first version's loads): W and the (cT) W of the orbit's
+  // cubes, the V^bc, V^ac, V^ab blocks of the triple, and (wave 0) t at
+  // (a,x), (b,x), (c,x) and eps_i for x in the blocks B1, B2, B3.
   double vals[HJ ? 2 : 1][3][6][2];
 #pragma unroll
   for (int set = 0; set < (HJ ? 2 : 1); set++)
@@ -613,31 +604,66 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
       for (int q = 0; q < 6; q++)
 #pragma unroll
         for (int h = 0; h < 2; h++)
-          if (q < nc) vals[set][cls][q][h] = gather(set ? GJ : G, q, h, cls);
+          vals[set][cls][q][h] = gather(set ? GJ : G, q, h, cls);
+  double vv[5];
+  {
+    const double *Vbc = P.VHH + (b + Nv * c) * N2;
+    const double *Vac = P.VHH + (a + Nv * c) * N2;
+    const double *Vab = P.VHH + (a + Nv * b) * N2;
+    const int Bp[3] = {B1, B2, B3};
+#pragma unroll
+    for (int it = 0; it < 5; it++) {
+      // entry e: slice v, block pair ps = 2 r + (s - [s > r]) of positions
+      // r != s, element l = ly + 8 lz
+      const int e = tid + 256 * it;
+      const int v = e / 384, ps = (e / 64) % 6, l = e & 63;
+      const int r = ps >> 1, s2 = (ps & 1) < r ? (ps & 1) : (ps & 1) + 1;
+      const int y = Bp[r] * 8 + (l & 7), z = Bp[s2] * 8 + (l >> 3);
+      const double *V = v == 0 ? Vbc : v == 1 ? Vac : Vab;
+      const bool ok_ = e < 3 * 6 * 64 && y < No && z < No;
+      const double val = V[ok_ ? y + No * z : 0];
+      vv[it] = ok_ ? val : 0.0;
+    }
+  }
+  if (tid < 64) {  // wave 0
+    const int l = tid < 24 ? tid : 0;
+    const int blk = l >> 3 == 0 ? B1 : l >> 3 == 1 ? B2 : B3;
+    const int x = blk * 8 + (l & 7);
+    const bool in = tid < 24 && x < no;
+    const int cx = x < no ? x : 0;
+    const double ta = P.Tai[a + Nv * cx], tb = P.Tai[b + Nv * cx],
+                 tc = P.Tai[c + Nv * cx], ex = P.ei[cx];
+    tA[tid] = in ? P.tai_sign * ta : 0.0;
+    tB[tid] = in ? P.tai_sign * tb : 0.0;
+    tC[tid] = in ? P.tai_sign * tc : 0.0;
+    sE[tid] = ex;
+  }
+#pragma unroll
+  for (int it = 0; it < 5; it++) sV[tid + 256 * it] = vv[it];
+  // the three class phases into LDS in the order w, v, u
 #pragma unroll
   for (int cls = 0; cls < 3; cls++) {
 #pragma unroll
     for (int q = 0; q < 6; q++)
 #pragma unroll
-      for (int h = 0; h < 2; h++)
-        if (q < nc) {
-          int at;
-          (void)elem(q, h, cls, at);
-          if (cls == 0) {
-            cube[q][at] = vals[0][0][q][h];
-            if constexpr (HJ) cubeJ[q][at] = vals[HJ ? 1 : 0][0][q][h];
-          } else {
-            cube[q][at] += vals[0][cls][q][h];
-            if constexpr (HJ) cubeJ[q][at] += vals[HJ ? 1 : 0][cls][q][h];
-          }
+      for (int h = 0; h < 2; h++) {
+        int at;
+        (void)elem(q, h, cls, at);
+        if (cls == 0) {
+          cube[q][at] = vals[0][0][q][h];
+          if constexpr (HJ) cubeJ[q][at] = vals[HJ ? 1 : 0][0][q][h];
+        } else {
+          cube[q][at] += vals[0][cls][q][h];
+          if constexpr (HJ) cubeJ[q][at] += vals[HJ ? 1 : 0][cls][q][h];
         }
+      }
     __syncthreads();
   }
 
   double e_acc = 0.0, ect_acc = 0.0;
-  triple_points<HJ, CZ, CSZ>(B1, B2, B3, no, distinct, epsabc, alias, cube,
-                             &cubeJ[0][0], tA, tB, tC, sE, sV, tid, e_acc,
-                             ect_acc);
+  triple_points<HJ, CZ, CSZ, true>(B1, B2, B3, no, distinct, epsabc, alias,
+                                   cube, &cubeJ[0][0], tA, tB, tC, sE, sV,
+                                   tid, e_acc, ect_acc);
   e_acc = wave_sum(e_acc);
   ect_acc = wave_sum(ect_acc);
   const int wave = tid >> 6, lane = tid & 63;
