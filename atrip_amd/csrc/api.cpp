@@ -748,7 +748,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   // on demand), or carved out of a fixed region without any allocation
   // (then the windows per launch shrink until the run fits).
   const int64_t nb3 = (int64_t)nb * nb * nb;
-  const int64_t W3 = s.No * s.No * s.No;
+  const int64_t W3 = g_elems(s.No, g32);  // one tuple's class buffer
   const bool use_j = s.has_j;
   const int nsets = s.cplx ? 4 : 1;
   const size_t gelem = g32 ? sizeof(float) : sizeof(double);
@@ -1310,7 +1310,7 @@ int atrip_hip_set_option(atrip_hip_ctx *ctx, const char *key, int64_t value) {
     } else if (k == "timing") {
       ctx->timing = value != 0;
     } else if (k == "tile") {
-      if (value < -1 || value > 7) throw Fail{"tile must be in -1..7"};
+      if (value < -1 || value > 9) throw Fail{"tile must be in -1..9"};
       ctx->tile = (int)value;
     } else if (k == "operand_store") {
       if (value < 0 || value > 3)

@@ -106,20 +106,24 @@ inline int pick_tile(int tile, int64_t No, int nb) {
 // Wave tiles of FM x FN fragments of 32 x 32 (2 x 2 waves per workgroup)
 // of the 32x32-MFMA kernels (bf16x3, split FP32), variant t of the "tile"
 // option.
-constexpr int N_T32_TILES = 8;
+constexpr int N_T32_TILES = 10;
 __host__ __device__ inline int t32_fm(int t) {
-  const int v[N_T32_TILES] = {2, 2, 3, 2, 3, 2, 3, 4};
+  const int v[N_T32_TILES] = {2, 2, 3, 2, 3, 2, 3, 4, 3, 4};
   return v[t];
 }
 __host__ __device__ inline int t32_fn(int t) {
-  const int v[N_T32_TILES] = {2, 3, 2, 4, 3, 5, 5, 4};
+  const int v[N_T32_TILES] = {2, 3, 2, 4, 3, 5, 5, 4, 4, 3};
   return v[t];
 }
-// Relative rate of each variant (operand bytes per MFMA cycle fall with
-// FM FN / (FM + FN)).
+// Rate of each variant relative to 4x4, per useful tile element: variants
+// 4-9 measured in bf16x3 on MI355X (mean of No=40, Nv=400 and No=60,
+// Nv=600, each divided by its padded-grid fraction); variants 0-3 from the
+// operand bytes per MFMA cycle, FM FN / (FM + FN), scaled by the ratio that
+// measured variants 4 and 5 show over the same formula.
 inline double t32_rate(int t) {
-  const double fm = t32_fm(t), fn = t32_fn(t);
-  return fm * fn / (fm + fn);
+  const double v[N_T32_TILES] = {0.62, 0.75, 0.75, 0.83, 0.985,
+                                 0.896, 0.988, 1.0, 0.989, 0.952};
+  return v[t];
 }
 // Automatic variant: rate times the useful fraction of the padded
 // M = No^2 by N = nb No tile grid.

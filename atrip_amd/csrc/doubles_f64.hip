@@ -42,7 +42,7 @@ struct DoublesParams {
   const int32_t *windows;
   const uint64_t *aprefix;
   double *G;
-  int64_t No, Nv, Kv, Ko, N2, W3, slots;
+  int64_t No, Nv, Kv, Ko, N2, W3, NoP, slots;
   uint64_t first, last;
   int nb, mtiles, ntiles;
   int64_t total, per_xcd;
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
         const int64_t m =
             (int64_t)mt * MT + wm * (MT / 2) + f * 16 + 4 * r + kk;
         if (m >= N2) continue;
-        Gs[m * No + rr] = acc[f][g][r];  // GEMM order: 128-byte rows
+        Gs[m * P.NoP + rr] = acc[f][g][r];  // GEMM order: contiguous rows
       }
     }
   }
@@ -230,7 +230,8 @@ hipError_t launch_doubles_f64(const DevSystem &s, const BatchArgs &b,
   P.Kv = s.Kv;
   P.Ko = s.Ko;
   P.N2 = s.No * s.No;
-  P.W3 = s.No * s.No * s.No;
+  P.NoP = g_pitch(s.No, false);
+  P.W3 = g_elems(s.No, false);
   P.slots = b.slots;
   P.first = b.first;
   P.last = b.last;

@@ -39,7 +39,7 @@ struct SplitParams {
   const int32_t *windows;
   const uint64_t *aprefix;
   float *G;
-  int64_t No, Nv, Kv, Ko, N2, W3, slots;
+  int64_t No, Nv, Kv, Ko, N2, W3, NoP, slots;
   uint64_t first, last;
   int nb, mtiles, ntiles;
   int64_t total, per_xcd;
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
         const int64_t m = (int64_t)mt * MT + wm * (MT / 2) + f * 32 +
                           8 * (r >> 2) + 4 * kh + (r & 3);
         if (m >= N2) continue;
-        Gs[m * No + rr] = acc[f][g][r];  // GEMM order: 128-byte rows
+        Gs[m * P.NoP + rr] = acc[f][g][r];  // GEMM order: contiguous rows
       }
     }
   }
@@ -261,7 +261,8 @@ hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
   P.Kv = s.Kv;
   P.Ko = s.Ko;
   P.N2 = s.No * s.No;
-  P.W3 = s.No * s.No * s.No;
+  P.NoP = g_pitch(s.No, true);
+  P.W3 = g_elems(s.No, true);
   P.slots = b.slots;
   P.first = b.first;
   P.last = b.last;
@@ -285,7 +286,7 @@ hipError_t launch_doubles_split(const DevSystem &s, const BatchArgs &b,
                          dim3((unsigned)grid), dim3(256), 0, st, P);         \
   }
   LAUNCH(2, 2) LAUNCH(2, 3) LAUNCH(3, 2) LAUNCH(2, 4) LAUNCH(3, 3)
-  LAUNCH(2, 5) LAUNCH(3, 5) LAUNCH(4, 4)
+  LAUNCH(2, 5) LAUNCH(3, 5) LAUNCH(4, 4) LAUNCH(3, 4) LAUNCH(4, 3)
 #undef LAUNCH
   return hipGetLastError();
 }

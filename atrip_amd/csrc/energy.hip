@@ -44,8 +44,9 @@ __device__ inline double wave_sum(double v) {
 template <typename GT>
 __global__ __launch_bounds__(256) void wsum_kernel(const GT *A, const GT *B,
                                                    double *out, int64_t No,
-                                                   int64_t W3g, int64_t slots,
-                                                   int nbk, int nbr) {
+                                                   int64_t NoP, int64_t W3g,
+                                                   int64_t slots, int nbk,
+                                                   int nbr) {
   // LDS brick [z][y][x] with pitch 17 (conflict-free transposed writes)
   __shared__ double br[16 * 17 * 17];
   const int64_t bid = blockIdx.x;
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(256) void wsum_kernel(const GT *A, const GT *B,
       const int e = tid + 256 * it;
       const int z = z0 + (e & 15), y = y0 + ((e >> 4) & 15), x = x0 + (e >> 8);
       const bool in = x < no && y < no && z < no;
-      v[it] = Sw[in ? (x + no * y) * No + z : 0];  // always a valid address
+      v[it] = Sw[in ? (x + no * y) * NoP + z : 0];  // always a valid address
       if (!in) v[it] = (GT)0;
     }
     // class v: (x + No z) No + y, y fastest
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(256) void wsum_kernel(const GT *A, const GT *B,
       const int e = tid + 256 * it;
       const int y = y0 + (e & 15), x = x0 + ((e >> 4) & 15), z = z0 + (e >> 8);
       const bool in = x < no && y < no && z < no;
-      t[it] = Sv[in ? (x + no * z) * No + y : 0];
+      t[it] = Sv[in ? (x + no * z) * NoP + y : 0];
       if (!in) t[it] = (GT)0;
     }
     __syncthreads();  // the previous set's cube writes are done with br
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(256) void wsum_kernel(const GT *A, const GT *B,
       const int e = tid + 256 * it;
       const int x = x0 + (e & 15), y = y0 + ((e >> 4) & 15), z = z0 + (e >> 8);
       const bool in = x < no && y < no && z < no;
-      v[it] = Su[in ? (y + no * z) * No + x : 0];
+      v[it] = Su[in ? (y + no * z) * NoP + x : 0];
       if (!in) v[it] = (GT)0;
     }
     __syncthreads();
@@ -452,7 +453,7 @@ struct FusedParams {
   const int32_t *windows;
   const uint64_t *aprefix;
   double *e_part;  // [slot][triple][2]
-  int64_t No, Nv, N2, W3g, slots, ntri;
+  int64_t No, Nv, N2, NoP, W3g, slots, ntri;
   uint64_t first, last;
   int nb, nbk;
   double tai_sign;
@@ -565,6 +566,7 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
   const GT *G = (const GT *)P.G;
   const GT *GJ = (const GT *)P.GJ;
   const int64_t W3g = P.W3g, slots = P.slots;
+  const int nop = (int)P.NoP;  // 32-bit: offsets use the 32x32->64 multiply-add
   const int64_t bw = gslot * W3g, bv = (slots + gslot) * W3g,
                 bu = (2 * slots + gslot) * W3g;
   // element (q, h) of class cls for this thread: its offset in the class
@@ -579,9 +581,9 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
     const int i = 8 * (qc[q] & 255) + x, j = 8 * ((qc[q] >> 8) & 255) + y,
               k = 8 * (qc[q] >> 16) + z;
     if (i >= no || j >= no || k >= no) return -1;
-    return cls == 0 ? bw + (int64_t)(i + no * j) * no + k      // w: k fastest
-         : cls == 1 ? bv + (int64_t)(i + no * k) * no + j      // v: j fastest
-                    : bu + (int64_t)(j + no * k) * no + i;     // u: i fastest
+    return cls == 0 ? bw + (int64_t)(i + no * j) * nop + k      // w: k fastest
+         : cls == 1 ? bv + (int64_t)(i + no * k) * nop + j      // v: j fastest
+                    : bu + (int64_t)(j + no * k) * nop + i;     // u: i fastest
   };
   auto gather = [&](const GT *S, int q, int h, int cls) -> double {
     int pos;
@@ -746,15 +748,15 @@ hipError_t launch_wsum(const DevSystem &s, const BatchArgs &b, const void *A,
   const int nbk = (int)w_blocks(s.No), nbr = (nbk + 1) / 2;
   const int64_t grid = b.slots * (int64_t)nbr * nbr * nbr;
   if (grid <= 0) return hipSuccess;
-  const int64_t W3g = s.No * s.No * s.No;
+  const int64_t NoP = g_pitch(s.No, g32), W3g = g_elems(s.No, g32);
   if (g32)
     hipLaunchKernelGGL(wsum_kernel<float>, dim3((unsigned)grid), dim3(256), 0,
-                       st, (const float *)A, (const float *)B, out, s.No, W3g,
+                       st, (const float *)A, (const float *)B, out, s.No, NoP, W3g,
                        b.slots, nbk, nbr);
   else
     hipLaunchKernelGGL(wsum_kernel<double>, dim3((unsigned)grid), dim3(256),
                        0, st, (const double *)A, (const double *)B, out, s.No,
-                       W3g, b.slots, nbk, nbr);
+                       NoP, W3g, b.slots, nbk, nbr);
   return hipGetLastError();
 }
 
@@ -777,7 +779,8 @@ hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
     P.No = s.No;
     P.Nv = s.Nv;
     P.N2 = s.No * s.No;
-    P.W3g = s.No * s.No * s.No;
+    P.NoP = g_pitch(s.No, g32);
+    P.W3g = g_elems(s.No, g32);
     P.slots = b.slots;
     P.ntri = ntri;
     P.first = b.first;

@@ -52,11 +52,23 @@ __host__ __device__ inline int tri_lane(int64_t R, int64_t k) {
 
 // Layout of the per-tuple class buffers written by the doubles kernels and
 // read by the energy kernels: each class keeps its GEMM output order,
-// element (row m, column index r) at m * No + r, so a 16-column MFMA output
-// fragment row is one contiguous 128-byte store.  In W indices:
-//   class w: m = i + No j, r = k   ->  (i + No j) No + k
-//   class v: m = i + No k, r = j   ->  (i + No k) No + j
-//   class u: m = j + No k, r = i   ->  (j + No k) No + i
+// element (row m, column index r) at m * P + r with the row pitch
+// P = g_pitch(No), so a 16-column MFMA output fragment row is one contiguous
+// store and every row starts on a 64-byte boundary.  In W indices:
+//   class w: m = i + No j, r = k   ->  (i + No j) P + k
+//   class v: m = i + No k, r = j   ->  (i + No k) P + j
+//   class u: m = j + No k, r = i   ->  (j + No k) P + i
+// Row pitch: No rounded up to 8 elements for FP64 buffers, 16 for FP32 (the
+// FP32-product forms), so the 64-byte row pieces the epilogue gathers never
+// straddle a line; the pad columns are never written or read.
+__host__ __device__ inline int64_t g_pitch(int64_t No, bool g32) {
+  const int64_t q = g32 ? 16 : 8;
+  return (No + q - 1) / q * q;
+}
+// Elements of one tuple's class buffer.
+__host__ __device__ inline int64_t g_elems(int64_t No, bool g32) {
+  return No * No * g_pitch(No, g32);
+}
 // The energy kernels stage 8x8x8 blocks of W in LDS (edge WCUBE).
 constexpr int WCUBE = 8;
 __host__ __device__ inline int64_t w_blocks(int64_t No) {
