@@ -124,12 +124,22 @@ def _sized_sample(O, no, nv, t, threads, budget_s):
     The oracle's one-time setup per run (cutting the T(x) slices of the
     whole Tabij) is timed with an empty range and taken out of the rate."""
     setup, _ = _timed_oracle(O, no, nv, t, 0, 0, threads)
-    probe = max(2 * threads, 4)
-    dt, _ = _timed_oracle(O, no, nv, t, 0, probe, threads)
-    per = max(dt - setup, 1e-4) / probe
+    log(f"CPU baseline {threads} thread(s): setup {setup:.1f} s")
+    # the probe grows until its tuple loop is well above the setup's noise,
+    # so the rate it gives cannot be a noise artefact
+    probe, loop = 4 * threads, 0.0
+    while True:
+        dt, _ = _timed_oracle(O, no, nv, t, 0, probe, threads)
+        loop = dt - setup
+        log(f"CPU baseline {threads} thread(s): probe {probe} tuples {dt:.1f} s")
+        if loop >= max(1.0, 0.1 * budget_s) or probe >= n_tuples(nv):
+            break
+        probe = min(n_tuples(nv), 4 * probe)
+    per = loop / probe
     count = int(max(probe, min(n_tuples(nv), budget_s / per)))
     count = max(threads, count // threads * threads)
     dt, _ = _timed_oracle(O, no, nv, t, 0, count, threads)
+    log(f"CPU baseline {threads} thread(s): {count} tuples {dt:.1f} s")
     return count, max(dt - setup, 1e-6), setup
 
 
@@ -150,12 +160,14 @@ def cpu_baseline(no: int, nv: int, seed: int, budget_s: float):
     t1 = O.synthetic(10, 50, 1)
     for label, th in (("1_core", 1), ("all_cores", cores)):
         dt, e = _timed_oracle(O, 10, 50, t1, 0, n_tuples(50), th)
+        log(f"CPU baseline No=10 Nv=50 on {th} thread(s): {dt:.1f} s")
         c1[label] = {"threads": th, "seconds": dt, "tuples_per_s": n_tuples(50) / dt,
                      "energy_Eh": e}
     del t1
     t0 = time.time()
     t = O.synthetic(no, nv, seed, nthreads=cores)
     gen_s = time.time() - t0
+    log(f"CPU baseline No={no} Nv={nv} host tensors generated in {gen_s:.1f} s")
     samples = {}
     for label, th in (("1_core", 1), ("all_cores", cores)):
         count, dt, setup = _sized_sample(O, no, nv, t, th, budget_s)
