@@ -483,9 +483,15 @@ __device__ __forceinline__ void triple_of(int t, int nbk, int &B1, int &B2,
 
 template <typename GT, bool HJ>
 __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
-  const int64_t gid = blockIdx.x;
-  const int64_t gslot = gid / P.ntri;
-  const int tri = (int)(gid % P.ntri);
+  // workgroups go to the 8 XCDs round-robin: XCD x runs the triples of
+  // slots x, x + 8, x + 16, ... one slot after the other, so the triples of
+  // a slot (whose row pieces share 128-byte lines) meet in one L2, and the
+  // slots stay spread over the XCDs
+  const int64_t bid = blockIdx.x, r = bid / 8;
+  const int64_t gslot = 8 * (r / P.ntri) + bid % 8;
+  const int tri = (int)(r % P.ntri);
+  if (gslot >= P.slots) return;
+  const int64_t gid = gslot * P.ntri + tri;
   const int nb = P.nb;
   const int64_t nb3 = (int64_t)nb * nb * nb;
   const int64_t w = gslot / nb3, s = gslot % nb3;
@@ -788,7 +794,7 @@ hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
     P.nb = b.nb;
     P.nbk = (int)nbk;
     P.tai_sign = tai_sign;
-    const int64_t grid = b.slots * ntri;
+    const int64_t grid = (b.slots + 7) / 8 * 8 * ntri;
     if (grid <= 0) return hipSuccess;
     if (g32 && GJ)
       hipLaunchKernelGGL((energy_fused_kernel<float, true>), dim3((unsigned)grid),
