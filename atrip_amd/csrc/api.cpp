@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "doubles_common.hpp"
 #include "host.hpp"
 #include "../../include/atrip_synth.h"
 
@@ -588,11 +589,12 @@ struct HostIndex {
 // Window plan for a system at the context's window edge: the block triples
 // in lexicographic order with their canonical index span and tuple count.
 WindowPlan &get_plan(atrip_hip_ctx *ctx, int sys, const DevSystem &s) {
+  const int nb_want = window_edge(ctx, s);
   auto it = ctx->plans.find(sys);
-  if (it != ctx->plans.end() && it->second.nb == ctx->window) return it->second;
+  if (it != ctx->plans.end() && it->second.nb == nb_want) return it->second;
   if (it != ctx->plans.end()) ctx->plans.erase(it);
   WindowPlan p;
-  p.nb = ctx->window;
+  p.nb = nb_want;
   p.nblk = (int)((s.Nv + p.nb - 1) / p.nb);
   const HostIndex hidx((uint64_t)s.Nv);
   const int nb = p.nb;
@@ -980,6 +982,10 @@ struct NoSink {
 
 }  // namespace
 
+int atrip_amd::window_edge(const atrip_hip_ctx *ctx, const DevSystem &s) {
+  return ctx->window > 0 ? ctx->window : auto_window(s.No, s.Nv);
+}
+
 uint64_t atrip_amd::plan_windows(atrip_hip_ctx *ctx, int sys) {
   DevSystem &s = get_sys(ctx, sys);
   return get_plan(ctx, sys, s).windows.size() / 3;
@@ -1302,7 +1308,8 @@ int atrip_hip_set_option(atrip_hip_ctx *ctx, const char *key, int64_t value) {
   return guarded(ctx, [&] {
     const std::string k = key ? key : "";
     if (k == "window") {
-      if (value < 1 || value > 32) throw Fail{"window must be in 1..32"};
+      if (value < 0 || value > 32)
+        throw Fail{"window must be in 0..32 (0: automatic)"};
       ctx->window = (int)value;
     } else if (k == "batch") {
       if (value < 1 || value > 4096) throw Fail{"batch must be in 1..4096"};

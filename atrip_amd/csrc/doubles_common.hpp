@@ -103,6 +103,38 @@ inline int pick_tile(int tile, int64_t No, int nb) {
   return pick;
 }
 
+// Automatic window edge of the plain path: the doubles GEMMs of a window
+// have N = nb No columns, so the edge sets how much of the last column tile
+// is padding.  Each candidate edge is scored by its best tile's modelled
+// rate (pick_tile's model) times the share of valid tuples in the windows;
+// an edge other than 8 is taken only when it scores 15 % above 8 (the
+// model's own error).  Measured in FP64 on MI355X: No=26, Nv=308 runs at
+// 40.6 TFLOP/s with edge 6 against 33.2 with edge 8; No=40 and No=60 keep 8.
+inline int auto_window(int64_t No, int64_t Nv) {
+  auto score = [&](int nb) {
+    const int t = pick_tile(-1, No, nb);
+    const double mt = 32.0 * tile_fm(t), nt = 32.0 * tile_fn(t);
+    const double M = (double)No * No, N = (double)nb * No;
+    const double use = M / (__builtin_ceil(M / mt) * mt) *
+                       N / (__builtin_ceil(N / nt) * nt);
+    const double nblk = __builtin_ceil((double)Nv / nb);
+    const double cells = nblk * (nblk + 1) * (nblk + 2) / 6 * nb * nb * nb;
+    const double tuples = (double)Nv * (Nv + 1) * (Nv + 2) / 6;
+    return tile_rate(t) * use * tuples / cells;
+  };
+  const double base = score(8);
+  int pick = 8;
+  double best = 1.15 * base;
+  for (int nb : {6, 12}) {
+    const double s = score(nb);
+    if (s > best) {
+      best = s;
+      pick = nb;
+    }
+  }
+  return pick;
+}
+
 // Wave tiles of FM x FN fragments of 32 x 32 (2 x 2 waves per workgroup)
 // of the 32x32-MFMA kernels (bf16x3, split FP32), variant t of the "tile"
 // option.

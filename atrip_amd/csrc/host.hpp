@@ -70,6 +70,9 @@ void run_fixed(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
 
 // Windows of a system's plan at the context's window edge (lexicographic
 // block triples holding a tuple).
+// Window edge of the plain path for system s: the "window" option, or
+// when it is 0 the automatic edge of the system's shape.
+int window_edge(const atrip_hip_ctx *ctx, const DevSystem &s);
 uint64_t plan_windows(atrip_hip_ctx *ctx, int sys);
 // Plain-path run (doubles kernel, energy epilogue) over plan windows
 // [wbegin, wend) of a system, every tuple of each, in launches of the

@@ -214,7 +214,7 @@ struct atrip_hip_ctx {
   std::map<int, atrip_amd::WindowPlan> plans;  // keyed by system
   int next_sys = 1;
   // options
-  int window = 8;
+  int window = 0;  // window edge; 0: automatic per system (auto_window)
   int batch = 4;
   int timing = 0;
   int tile = -1;  // -1: chosen per system shape

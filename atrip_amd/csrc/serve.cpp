@@ -169,9 +169,9 @@ void admit(atrip_hip_server *srv, int grid) {
     r.plain = runs_plain(srv, r.No);
     if (r.plain) {
       r.nwin = plan_windows(ctx, sys);
-      r.plan.nb = ctx->window;
+      r.plan.nb = window_edge(ctx, get_sys(ctx, sys));
       r.plan.measured = false;
-      r.plan.sec_window = plain_window_estimate(r.No, r.Nv, ctx->window,
+      r.plan.sec_window = plain_window_estimate(r.No, r.Nv, r.plan.nb,
                                                 srv->precision, grid);
     } else {
       r.plan = plan_for(ctx, get_sys(ctx, sys), grid, srv->precision);

@@ -256,6 +256,8 @@ def bench_serving(ctx, n_requests: int, rate: float, seed: int,
 
     out = {}
     ref = None
+    window = ctx.get_option("window")
+    ctx.set_option("window", 0)  # plain-path requests: edge per request shape
     for prec in precisions:
         log(f"serving form, {prec}")
         r = serve.run_poisson(ctx, n_requests, rate=rate, seed=seed,
@@ -265,14 +267,16 @@ def bench_serving(ctx, n_requests: int, rate: float, seed: int,
         r["workload"] = (f"{n_requests} requests, Poisson {rate} req/s, No log-U[8,40], "
                          f"Nv log-U[40,320], seed {seed}, tensor seeds 5000+r; "
                          f"HBM budget 200 GB; {prec}; requests with No >= "
-                         f"{ctx.get_option('serve_plain_no')} on the plain path, "
-                         f"the rest in the persistent batched launch")
+                         f"{ctx.get_option('serve_plain_no')} on the plain path "
+                         f"(window edge per request shape), the rest in the "
+                         f"persistent batched launch")
         if ref is None:
             ref = e
         else:
             rel = [abs(e[q] - ref[q]) / abs(ref[q]) for q in e if q in ref and ref[q]]
             r["max_rel_dE_vs_fp64"] = max(rel) if rel else None
         out[prec] = r
+    ctx.set_option("window", window)
     return out
 
 

@@ -206,7 +206,9 @@ int atrip_hip_tuple_energies(atrip_hip_ctx *ctx, int sys, int precision,
                              double *e, double *e_ct);
 
 /* Tuning and instrumentation knobs (no counterpart in the reference):
- *   "window"  block edge nb of the tuple windows (1..32)
+ *   "window"  block edge nb of the tuple windows (1..32); 0 (default):
+ *             chosen per system shape (8 unless a modelled tile fit of
+ *             edge 6 or 12 is 15 % better)
  *   "batch"   windows per launch
  *   "timing"  1: time every hot-kernel launch with HIP events
  *   "tile"    doubles-kernel tile variant 0..7, -1 (default) = automatic

@@ -118,14 +118,14 @@ def test_partition_independence(hip_ctx, oracle):
     try:
         n = oracle.n_tuples(nv)
         outs = []
-        for window, batch, tile in [(8, 4, 0), (4, 1, 0), (16, 2, 1), (5, 7, 2), (8, 2, 5),
+        for window, batch, tile in [(8, 4, 0), (4, 1, 0), (16, 2, 1), (5, 7, 2), (8, 2, 5), (0, 4, -1),
                                     (3, 5, 3), (8, 1, -1)]:
             hip_ctx.set_option("window", window)
             hip_ctx.set_option("batch", batch)
             hip_ctx.set_option("tile", tile)
             outs.append(hip_ctx.tuple_energies(s, 0, n))
         hip_ctx.set_option("tile", -1)
-        hip_ctx.set_option("window", 8)
+        hip_ctx.set_option("window", 0)
         hip_ctx.set_option("batch", 4)
         # a single tuple alone gives the same bits as in the full run
         alone = hip_ctx.tuple_energies(s, 1234, 1)
@@ -175,7 +175,7 @@ def test_benchmark_size_sample_and_properties(hip_ctx, oracle):
             hip_ctx.set_option("tile", tile)
             assert np.array_equal(hip_ctx.tuple_energies(s, first, count), base)
         hip_ctx.set_option("tile", -1)
-        hip_ctx.set_option("window", 8)
+        hip_ctx.set_option("window", 0)
         hip_ctx.set_option("batch", 4)
         for precision in MIXED_FORMS:
             print("benchmark size:", precision, flush=True)
@@ -211,7 +211,7 @@ def test_errors_are_reported(hip_ctx):
     with pytest.raises(hip.HipError, match="unknown system"):
         hip_ctx.run(987654)
     with pytest.raises(hip.HipError, match="window"):
-        hip_ctx.set_option("window", 0)
+        hip_ctx.set_option("window", -1)
     with pytest.raises(hip.HipError, match="unknown option"):
         hip_ctx.set_option("no-such-knob", 1)
 
@@ -265,7 +265,7 @@ def test_mixed_precision_cT_and_partition(hip_ctx, oracle, precision):
         hip_ctx.set_option("window", 4)
         hip_ctx.set_option("batch", 1)
         b = hip_ctx.tuple_energies(s, 0, n, precision=precision)
-        hip_ctx.set_option("window", 8)
+        hip_ctx.set_option("window", 0)
         hip_ctx.set_option("batch", 4)
     finally:
         hip_ctx.release(s)
@@ -339,7 +339,7 @@ def test_mixed_tiles_and_windows(hip_ctx, oracle, precision):
             outs.append(hip_ctx.tuple_energies(s, 0, n, precision=precision))
     finally:
         hip_ctx.set_option("tile", -1)
-        hip_ctx.set_option("window", 8)
+        hip_ctx.set_option("window", 0)
         hip_ctx.set_option("batch", 4)
         hip_ctx.release(s)
     for o in outs[1:]:
@@ -463,5 +463,5 @@ def test_fused_and_cube_epilogues_agree(hip_ctx):
         hip_ctx.release(s_)
     finally:
         hip_ctx.set_option("epilogue", 0)
-        hip_ctx.set_option("window", 8)
+        hip_ctx.set_option("window", 0)
         hip_ctx.set_option("batch", 4)
