@@ -588,8 +588,9 @@ struct HostIndex {
 
 // Window plan for a system at the context's window edge: the block triples
 // in lexicographic order with their canonical index span and tuple count.
-WindowPlan &get_plan(atrip_hip_ctx *ctx, int sys, const DevSystem &s) {
-  const int nb_want = window_edge(ctx, s);
+WindowPlan &get_plan(atrip_hip_ctx *ctx, int sys, const DevSystem &s,
+                     int nb_want = 0) {
+  if (nb_want <= 0) nb_want = window_edge(ctx, s);
   auto it = ctx->plans.find(sys);
   if (it != ctx->plans.end() && it->second.nb == nb_want) return it->second;
   if (it != ctx->plans.end()) ctx->plans.erase(it);
@@ -668,6 +669,7 @@ struct FixedScratch {
 struct WindowSel {
   uint64_t stride = 1, offset = 0;
   uint64_t wbegin = 0, wend = ~0ULL;
+  int nb = 0;  // window edge of the plan; 0: the context's (window_edge)
 };
 
 template <typename Sink>
@@ -677,7 +679,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
                     const FixedScratch *fixed = nullptr) {
   drop_pending_events(ctx);
   DevSystem &s = get_sys(ctx, sys);
-  WindowPlan &plan = get_plan(ctx, sys, s);
+  WindowPlan &plan = get_plan(ctx, sys, s, wsel.nb);
   const uint64_t n = n_tuples(s.Nv);
   last = std::min(last, n);
   RunResult res;
@@ -986,13 +988,13 @@ int atrip_amd::window_edge(const atrip_hip_ctx *ctx, const DevSystem &s) {
   return ctx->window > 0 ? ctx->window : auto_window(s.No, s.Nv);
 }
 
-uint64_t atrip_amd::plan_windows(atrip_hip_ctx *ctx, int sys) {
+uint64_t atrip_amd::plan_windows(atrip_hip_ctx *ctx, int sys, int nb) {
   DevSystem &s = get_sys(ctx, sys);
-  return get_plan(ctx, sys, s).windows.size() / 3;
+  return get_plan(ctx, sys, s, nb).windows.size() / 3;
 }
 
 void atrip_amd::run_plan_windows(atrip_hip_ctx *ctx, int sys, int precision,
-                                 uint64_t wbegin, uint64_t wend,
+                                 int nb, uint64_t wbegin, uint64_t wend,
                                  std::vector<double> &out) {
   out.clear();
   struct WinSink {
@@ -1017,6 +1019,7 @@ void atrip_amd::run_plan_windows(atrip_hip_ctx *ctx, int sys, int precision,
   WindowSel ws;
   ws.wbegin = wbegin;
   ws.wend = wend;
+  ws.nb = nb;
   run_range(ctx, sys, precision, 0, 0, ~0ULL, sink, ws);
 }
 

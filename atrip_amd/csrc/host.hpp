@@ -73,14 +73,15 @@ void run_fixed(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
 // Window edge of the plain path for system s: the "window" option, or
 // when it is 0 the automatic edge of the system's shape.
 int window_edge(const atrip_hip_ctx *ctx, const DevSystem &s);
-uint64_t plan_windows(atrip_hip_ctx *ctx, int sys);
-// Plain-path run (doubles kernel, energy epilogue) over plan windows
-// [wbegin, wend) of a system, every tuple of each, in launches of the
-// context's "batch" windows.  out receives per window, in plan order, the
+// Windows of the plain-path plan of edge nb of a system.
+uint64_t plan_windows(atrip_hip_ctx *ctx, int sys, int nb);
+// Plain-path run (doubles kernel, energy epilogue) over windows
+// [wbegin, wend) of the plan of edge nb of a system, every tuple of each,
+// in launches of the context's "batch" windows.  out receives per window, in plan order, the
 // (sum, comp) of a compensated sum of its raw tuple energies in slot order,
 // so a window's pair does not depend on how windows were grouped into
 // launches or calls.
-void run_plan_windows(atrip_hip_ctx *ctx, int sys, int precision,
+void run_plan_windows(atrip_hip_ctx *ctx, int sys, int precision, int nb,
                       uint64_t wbegin, uint64_t wend,
                       std::vector<double> &out);
 

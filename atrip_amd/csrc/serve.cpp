@@ -168,8 +168,10 @@ void admit(atrip_hip_server *srv, int grid) {
     srv->peak_resident = std::max(srv->peak_resident, srv->resident);
     r.plain = runs_plain(srv, r.No);
     if (r.plain) {
-      r.nwin = plan_windows(ctx, sys);
+      // the request keeps the window edge it was admitted with, whatever
+      // the context's option does later
       r.plan.nb = window_edge(ctx, get_sys(ctx, sys));
+      r.nwin = plan_windows(ctx, sys, r.plan.nb);
       r.plan.measured = false;
       r.plan.sec_window = plain_window_estimate(r.No, r.Nv, r.plan.nb,
                                                 srv->precision, grid);
@@ -268,7 +270,8 @@ int step(atrip_hip_server *srv) {
   for (auto &pk : plain) {
     Request &r = srv->reqs.at(pk.first);
     const Clock::time_point t0 = Clock::now();
-    run_plan_windows(ctx, r.sys, srv->precision, r.cursor, r.cursor + pk.second,
+    run_plan_windows(ctx, r.sys, srv->precision, r.plan.nb, r.cursor,
+                     r.cursor + pk.second,
                      wsum);
     const double ms =
         std::chrono::duration<double, std::milli>(Clock::now() - t0).count();

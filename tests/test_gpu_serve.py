@@ -203,6 +203,32 @@ def test_plain_path_routing(hip_ctx, alone, plain_no):
         assert _close(solo[m], alone[m[2]][1], REL_PLAIN), m
 
 
+def test_serving_keeps_window_edge_of_admission(hip_ctx):
+    """A plain-path request keeps the window edge it was admitted with: the
+    context's window option changed in mid-stream leaves every result
+    bitwise the same as served alone at the edge of admission."""
+    print("serve window edge pinned", flush=True)
+    mix = MIX[:4]
+    hip_ctx.set_option("serve_plain_no", 1)
+    try:
+        solo = {}
+        for m in mix:
+            with serve.Server(hip_ctx, hbm_budget_gb=50) as srv:
+                rid = srv.submit_synthetic(*m)
+                solo[m] = srv.drain([rid])[rid]["energy"]
+        with serve.Server(hip_ctx, hbm_budget_gb=50, iter_ms=2.0) as srv:
+            ids = [srv.submit_synthetic(*m) for m in mix]
+            srv.step()  # admits all four at the automatic edge
+            hip_ctx.set_option("window", 4)
+            res = srv.drain(ids)
+    finally:
+        hip_ctx.set_option("window", 0)
+        hip_ctx.set_option("serve_plain_no", 16)
+    for rid, m in zip(ids, mix):
+        assert res[rid]["state"] == serve.DONE
+        assert res[rid]["energy"] == solo[m], m
+
+
 def test_bf16x3_serving(hip_ctx):
     """A bf16x3 server (every request on the plain path, bf16x3 operands
     only): each result bitwise the same alone and in the mix, equal to the
