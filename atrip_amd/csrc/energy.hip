@@ -487,15 +487,22 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
   // slots x, x + 8, x + 16, ... one slot after the other, so the triples of
   // a slot (whose row pieces share 128-byte lines) meet in one L2, and the
   // slots stay spread over the XCDs
-  const int64_t bid = blockIdx.x, r = bid / 8;
-  const int64_t gslot = 8 * (r / P.ntri) + bid % 8;
-  const int tri = (int)(r % P.ntri);
-  if (gslot >= P.slots) return;
+  // (all of the decode in 32-bit unsigned arithmetic: a 64-bit division is
+  // ~130 scalar instructions, and the scalar unit, shared by the CU's
+  // waves, was the kernel's bound)
+  const uint32_t bid = blockIdx.x, r = bid >> 3, ntri = (uint32_t)P.ntri;
+  const uint32_t rq = r / ntri;
+  const uint32_t gs = 8 * rq + (bid & 7);
+  const int tri = (int)(r - rq * ntri);
+  if ((int64_t)gs >= P.slots) return;
+  const int64_t gslot = gs;
   const int64_t gid = gslot * P.ntri + tri;
   const int nb = P.nb;
-  const int64_t nb3 = (int64_t)nb * nb * nb;
-  const int64_t w = gslot / nb3, s = gslot % nb3;
-  const int64_t ta = s / (nb * nb), tb = (s / nb) % nb, tc = s % nb;
+  const uint32_t nb2 = (uint32_t)(nb * nb), nb3 = nb2 * (uint32_t)nb;
+  const uint32_t w32 = gs / nb3, s32 = gs - w32 * nb3;
+  const uint32_t ta32 = s32 / nb2, r2 = s32 - ta32 * nb2;
+  const uint32_t tb32 = r2 / (uint32_t)nb, tc32 = r2 - tb32 * (uint32_t)nb;
+  const int64_t w = w32, ta = ta32, tb = tb32, tc = tc32;
   const int64_t a = P.windows[3 * w] * (int64_t)nb + ta;
   const int64_t b = P.windows[3 * w + 1] * (int64_t)nb + tb;
   const int64_t c = P.windows[3 * w + 2] * (int64_t)nb + tc;
@@ -575,9 +582,12 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
   const int nop = (int)P.NoP;  // 32-bit: offsets use the 32x32->64 multiply-add
   const int64_t bw = gslot * W3g, bv = (slots + gslot) * W3g,
                 bu = (2 * slots + gslot) * W3g;
-  // element (q, h) of class cls for this thread: its offset in the class
-  // buffer (or -1 outside the occupied range) and its LDS position
-  auto elem = [&](int q, int h, int cls, int &pos) -> int64_t {
+  // element (q, h) of class cls for this thread: its offset inside the
+  // slot's class buffer (0 outside the occupied range, with in false) and
+  // its LDS position.  Branch-free, and 32-bit: the class base is
+  // wave-uniform, so a load is a scalar base plus a 32-bit lane offset
+  // (No^2 times the row pitch stays far below 2^32).
+  auto elem = [&](int q, int h, int cls, int &pos, bool &in) -> uint32_t {
     const int e = tid + 256 * h, r = e >> 3, f = e & 7;
     const int r0 = r & 7, r1 = r >> 3;
     const int x = cls == 2 ? f : r0;
@@ -586,18 +596,21 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
     pos = x + 8 * y + CZ * z;
     const int i = 8 * (qc[q] & 255) + x, j = 8 * ((qc[q] >> 8) & 255) + y,
               k = 8 * (qc[q] >> 16) + z;
-    if (i >= no || j >= no || k >= no) return -1;
-    return cls == 0 ? bw + (int64_t)(i + no * j) * nop + k      // w: k fastest
-         : cls == 1 ? bv + (int64_t)(i + no * k) * nop + j      // v: j fastest
-                    : bu + (int64_t)(j + no * k) * nop + i;     // u: i fastest
+    in = (i < no) & (j < no) & (k < no);
+    const uint32_t o =
+        cls == 0 ? (uint32_t)((i + no * j) * nop + k)      // w: k fastest
+      : cls == 1 ? (uint32_t)((i + no * k) * nop + j)      // v: j fastest
+                 : (uint32_t)((j + no * k) * nop + i);     // u: i fastest
+    return in ? o : 0u;  // always a valid address
   };
   auto gather = [&](const GT *S, int q, int h, int cls) -> double {
     int pos;
-    const int64_t off = elem(q, h, cls, pos);
-    const GT v = S[off >= 0 ? off : 0];  // always a valid address
-    return off >= 0 ? (double)v : 0.0;
+    bool in;
+    const uint32_t o = elem(q, h, cls, pos, in);
+    const GT *Sc = S + (cls == 0 ? bw : cls == 1 ? bv : bu);
+    const GT v = Sc[o];
+    return in ? (double)v : 0.0;
   };
-
   // Every global load of the workgroup is issued before any is waited on
   // (a load behind a branch is waited on before the branch joins, which
   // serialised the first version's loads): W and the (cT) W of the orbit's
@@ -628,7 +641,7 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
       const int r = ps >> 1, s2 = (ps & 1) < r ? (ps & 1) : (ps & 1) + 1;
       const int y = Bp[r] * 8 + (l & 7), z = Bp[s2] * 8 + (l >> 3);
       const double *V = v == 0 ? Vbc : v == 1 ? Vac : Vab;
-      const bool ok_ = e < 3 * 6 * 64 && y < No && z < No;
+      const bool ok_ = (e < 3 * 6 * 64) & (y < No) & (z < No);
       const double val = V[ok_ ? y + No * z : 0];
       vv[it] = ok_ ? val : 0.0;
     }
@@ -656,7 +669,8 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         int at;
-        (void)elem(q, h, cls, at);
+        bool in_;
+        (void)elem(q, h, cls, at, in_);
         if (cls == 0) {
           cube[q][at] = vals[0][0][q][h];
           if constexpr (HJ) cubeJ[q][at] = vals[HJ ? 1 : 0][0][q][h];
