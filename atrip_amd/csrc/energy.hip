@@ -15,6 +15,8 @@
 // spread over the workgroup; each thread sums its points in a fixed order
 // and a fixed shuffle/LDS tree combines the threads, so the per-tuple energy
 // does not depend on which other tuples share the launch.
+#include <type_traits>
+
 #include "internal.hpp"
 
 namespace atrip_amd {
@@ -481,6 +483,153 @@ __device__ __forceinline__ void triple_of(int t, int nbk, int &B1, int &B2,
   B1 = B2 + r;
 }
 
+// The loads, class phases and points of one block triple's orbit, whose
+// NC distinct cubes (6, 3 or 1) are qc[0 .. NC - 1]: one compile-time
+// instance per NC, so every load of an instance is issued before any is
+// waited on and none is spent on a repeated cube.
+template <typename GT, bool HJ, int NC, int CZ, int CSZ>
+__device__ __forceinline__ void fused_orbit(
+    const FusedParams &P, int64_t gid, int64_t gslot, int tid, int no,
+    int64_t a, int64_t b, int64_t c, int B1, int B2, int B3, int distinct,
+    double epsabc, const int (&qc)[6], const int (&alias)[6],
+    double (*cube)[CSZ], double (*cubeJ)[HJ ? CSZ : 1], double *tA,
+    double *tB, double *tC, double *sE, double *sV, double (*red)[4]) {
+  const int64_t No = P.No, Nv = P.Nv, N2 = P.N2;
+  // element e = tid + 256 h of a cube, per class: row e >> 3 of the class's
+  // two slower indices, f = e & 7 along its fastest index
+  const GT *G = (const GT *)P.G;
+  const GT *GJ = (const GT *)P.GJ;
+  const int64_t W3g = P.W3g, slots = P.slots;
+  const int nop = (int)P.NoP;  // 32-bit: offsets use the 32x32->64 multiply-add
+  const int64_t bw = gslot * W3g, bv = (slots + gslot) * W3g,
+                bu = (2 * slots + gslot) * W3g;
+  // element (q, h) of class cls for this thread: its offset inside the
+  // slot's class buffer (0 outside the occupied range, with in false) and
+  // its LDS position.  Branch-free, and 32-bit: the class base is
+  // wave-uniform, so a load is a scalar base plus a 32-bit lane offset
+  // (No^2 times the row pitch stays far below 2^32).
+  auto elem = [&](int q, int h, int cls, int &pos, bool &in) -> uint32_t {
+    const int e = tid + 256 * h, r = e >> 3, f = e & 7;
+    const int r0 = r & 7, r1 = r >> 3;
+    const int x = cls == 2 ? f : r0;
+    const int y = cls == 0 ? r1 : cls == 1 ? f : r0;
+    const int z = cls == 0 ? f : r1;
+    pos = x + 8 * y + CZ * z;
+    const int i = 8 * (qc[q] & 255) + x, j = 8 * ((qc[q] >> 8) & 255) + y,
+              k = 8 * (qc[q] >> 16) + z;
+    in = (i < no) & (j < no) & (k < no);
+    const uint32_t o =
+        cls == 0 ? (uint32_t)((i + no * j) * nop + k)      // w: k fastest
+      : cls == 1 ? (uint32_t)((i + no * k) * nop + j)      // v: j fastest
+                 : (uint32_t)((j + no * k) * nop + i);     // u: i fastest
+    return in ? o : 0u;  // always a valid address
+  };
+  auto gather = [&](const GT *S, int q, int h, int cls) -> double {
+    int pos;
+    bool in;
+    const uint32_t o = elem(q, h, cls, pos, in);
+    const GT *Sc = S + (cls == 0 ? bw : cls == 1 ? bv : bu);
+    const GT v = Sc[o];
+    return in ? (double)v : 0.0;
+  };
+  // The loads, class phases and points for the orbit's nc distinct cubes
+  // (6, 3 or 1), one compile-time instance each, selected by a uniform
+  // branch: every load of an instance is issued before any is waited on
+  // and none is spent on a repeated cube.
+  // Every global load of the workgroup is issued before any is waited on
+  // (a load behind a branch is waited on before the branch joins, which
+  // serialised the first version's loads): W and the (cT) W of the orbit's
+  // cubes, the V^bc, V^ac, V^ab blocks of the triple, and (wave 0) t at
+  // (a,x), (b,x), (c,x) and eps_i for x in the blocks B1, B2, B3.
+  double vals[HJ ? 2 : 1][3][NC][2];
+#pragma unroll
+  for (int set = 0; set < (HJ ? 2 : 1); set++)
+#pragma unroll
+    for (int cls = 0; cls < 3; cls++)
+#pragma unroll
+      for (int q = 0; q < NC; q++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+          vals[set][cls][q][h] = gather(set ? GJ : G, q, h, cls);
+  double vv[5];
+  {
+    const double *Vbc = P.VHH + (b + Nv * c) * N2;
+    const double *Vac = P.VHH + (a + Nv * c) * N2;
+    const double *Vab = P.VHH + (a + Nv * b) * N2;
+#pragma unroll
+    for (int it = 0; it < 5; it++) {
+      // entry e: slice v, block pair ps = 2 r + (s - [s > r]) of positions
+      // r != s, element l = ly + 8 lz
+      const int e = tid + 256 * it;
+      const int v = e / 384, ps = (e / 64) % 6, l = e & 63;
+      const int r = ps >> 1, s2 = (ps & 1) < r ? (ps & 1) : (ps & 1) + 1;
+      // (selects, not an array: a run-time-indexed array lives in
+      // scratch, and its loads count in vmcnt behind the gathers)
+      const int Br = r == 0 ? B1 : r == 1 ? B2 : B3;
+      const int Bs = s2 == 0 ? B1 : s2 == 1 ? B2 : B3;
+      const int y = Br * 8 + (l & 7), z = Bs * 8 + (l >> 3);
+      const double *V = v == 0 ? Vbc : v == 1 ? Vac : Vab;
+      const bool ok_ = (e < 3 * 6 * 64) & (y < No) & (z < No);
+      const double val = V[ok_ ? y + No * z : 0];
+      vv[it] = ok_ ? val : 0.0;
+    }
+  }
+  if (tid < 64) {  // wave 0
+    const int l = tid < 24 ? tid : 0;
+    const int blk = l >> 3 == 0 ? B1 : l >> 3 == 1 ? B2 : B3;
+    const int x = blk * 8 + (l & 7);
+    const bool in = tid < 24 && x < no;
+    const int cx = x < no ? x : 0;
+    const double ta = P.Tai[a + Nv * cx], tb = P.Tai[b + Nv * cx],
+                 tc = P.Tai[c + Nv * cx], ex = P.ei[cx];
+    tA[tid] = in ? P.tai_sign * ta : 0.0;
+    tB[tid] = in ? P.tai_sign * tb : 0.0;
+    tC[tid] = in ? P.tai_sign * tc : 0.0;
+    sE[tid] = ex;
+  }
+#pragma unroll
+  for (int it = 0; it < 5; it++) sV[tid + 256 * it] = vv[it];
+  // the three class phases into LDS in the order w, v, u
+#pragma unroll
+  for (int cls = 0; cls < 3; cls++) {
+#pragma unroll
+    for (int q = 0; q < NC; q++)
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        int at;
+        bool in_;
+        (void)elem(q, h, cls, at, in_);
+        if (cls == 0) {
+          cube[q][at] = vals[0][0][q][h];
+          if constexpr (HJ) cubeJ[q][at] = vals[HJ ? 1 : 0][0][q][h];
+        } else {
+          cube[q][at] += vals[0][cls][q][h];
+          if constexpr (HJ) cubeJ[q][at] += vals[HJ ? 1 : 0][cls][q][h];
+        }
+      }
+    __syncthreads();
+  }
+
+  double e_acc = 0.0, ect_acc = 0.0;
+  triple_points<HJ, CZ, CSZ, true>(B1, B2, B3, no, distinct, epsabc, alias,
+                                   cube, &cubeJ[0][0], tA, tB, tC, sE, sV,
+                                   tid, e_acc, ect_acc);
+  e_acc = wave_sum(e_acc);
+  ect_acc = wave_sum(ect_acc);
+  const int wave = tid >> 6, lane = tid & 63;
+  if (lane == 0) {
+    red[0][wave] = e_acc;
+    red[1][wave] = ect_acc;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const double e = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    const double ect = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    P.e_part[2 * gid] = e;
+    P.e_part[2 * gid + 1] = HJ ? ect : e;
+  }
+}
+
 template <typename GT, bool HJ>
 __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
   // workgroups go to the 8 XCDs round-robin: XCD x runs the triples of
@@ -567,139 +716,19 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
       nc++;
     }
   }
-  // slots past the distinct cubes repeat cube 0: their loads are issued
-  // like the others (none waits behind a branch) and hit in cache; what they
-  // write to LDS is never read
-#pragma unroll
-  for (int q = 1; q < 6; q++)
-    if (q >= nc) qc[q] = qc[0];
 
-  // element e = tid + 256 h of a cube, per class: row e >> 3 of the class's
-  // two slower indices, f = e & 7 along its fastest index
-  const GT *G = (const GT *)P.G;
-  const GT *GJ = (const GT *)P.GJ;
-  const int64_t W3g = P.W3g, slots = P.slots;
-  const int nop = (int)P.NoP;  // 32-bit: offsets use the 32x32->64 multiply-add
-  const int64_t bw = gslot * W3g, bv = (slots + gslot) * W3g,
-                bu = (2 * slots + gslot) * W3g;
-  // element (q, h) of class cls for this thread: its offset inside the
-  // slot's class buffer (0 outside the occupied range, with in false) and
-  // its LDS position.  Branch-free, and 32-bit: the class base is
-  // wave-uniform, so a load is a scalar base plus a 32-bit lane offset
-  // (No^2 times the row pitch stays far below 2^32).
-  auto elem = [&](int q, int h, int cls, int &pos, bool &in) -> uint32_t {
-    const int e = tid + 256 * h, r = e >> 3, f = e & 7;
-    const int r0 = r & 7, r1 = r >> 3;
-    const int x = cls == 2 ? f : r0;
-    const int y = cls == 0 ? r1 : cls == 1 ? f : r0;
-    const int z = cls == 0 ? f : r1;
-    pos = x + 8 * y + CZ * z;
-    const int i = 8 * (qc[q] & 255) + x, j = 8 * ((qc[q] >> 8) & 255) + y,
-              k = 8 * (qc[q] >> 16) + z;
-    in = (i < no) & (j < no) & (k < no);
-    const uint32_t o =
-        cls == 0 ? (uint32_t)((i + no * j) * nop + k)      // w: k fastest
-      : cls == 1 ? (uint32_t)((i + no * k) * nop + j)      // v: j fastest
-                 : (uint32_t)((j + no * k) * nop + i);     // u: i fastest
-    return in ? o : 0u;  // always a valid address
-  };
-  auto gather = [&](const GT *S, int q, int h, int cls) -> double {
-    int pos;
-    bool in;
-    const uint32_t o = elem(q, h, cls, pos, in);
-    const GT *Sc = S + (cls == 0 ? bw : cls == 1 ? bv : bu);
-    const GT v = Sc[o];
-    return in ? (double)v : 0.0;
-  };
-  // Every global load of the workgroup is issued before any is waited on
-  // (a load behind a branch is waited on before the branch joins, which
-  // serialised the first version's loads): W and the (cT) W of the orbit's
-  // cubes, the V^bc, V^ac, V^ab blocks of the triple, and (wave 0) t at
-  // (a,x), (b,x), (c,x) and eps_i for x in the blocks B1, B2, B3.
-  double vals[HJ ? 2 : 1][3][6][2];
-#pragma unroll
-  for (int set = 0; set < (HJ ? 2 : 1); set++)
-#pragma unroll
-    for (int cls = 0; cls < 3; cls++)
-#pragma unroll
-      for (int q = 0; q < 6; q++)
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-          vals[set][cls][q][h] = gather(set ? GJ : G, q, h, cls);
-  double vv[5];
-  {
-    const double *Vbc = P.VHH + (b + Nv * c) * N2;
-    const double *Vac = P.VHH + (a + Nv * c) * N2;
-    const double *Vab = P.VHH + (a + Nv * b) * N2;
-    const int Bp[3] = {B1, B2, B3};
-#pragma unroll
-    for (int it = 0; it < 5; it++) {
-      // entry e: slice v, block pair ps = 2 r + (s - [s > r]) of positions
-      // r != s, element l = ly + 8 lz
-      const int e = tid + 256 * it;
-      const int v = e / 384, ps = (e / 64) % 6, l = e & 63;
-      const int r = ps >> 1, s2 = (ps & 1) < r ? (ps & 1) : (ps & 1) + 1;
-      const int y = Bp[r] * 8 + (l & 7), z = Bp[s2] * 8 + (l >> 3);
-      const double *V = v == 0 ? Vbc : v == 1 ? Vac : Vab;
-      const bool ok_ = (e < 3 * 6 * 64) & (y < No) & (z < No);
-      const double val = V[ok_ ? y + No * z : 0];
-      vv[it] = ok_ ? val : 0.0;
-    }
-  }
-  if (tid < 64) {  // wave 0
-    const int l = tid < 24 ? tid : 0;
-    const int blk = l >> 3 == 0 ? B1 : l >> 3 == 1 ? B2 : B3;
-    const int x = blk * 8 + (l & 7);
-    const bool in = tid < 24 && x < no;
-    const int cx = x < no ? x : 0;
-    const double ta = P.Tai[a + Nv * cx], tb = P.Tai[b + Nv * cx],
-                 tc = P.Tai[c + Nv * cx], ex = P.ei[cx];
-    tA[tid] = in ? P.tai_sign * ta : 0.0;
-    tB[tid] = in ? P.tai_sign * tb : 0.0;
-    tC[tid] = in ? P.tai_sign * tc : 0.0;
-    sE[tid] = ex;
-  }
-#pragma unroll
-  for (int it = 0; it < 5; it++) sV[tid + 256 * it] = vv[it];
-  // the three class phases into LDS in the order w, v, u
-#pragma unroll
-  for (int cls = 0; cls < 3; cls++) {
-#pragma unroll
-    for (int q = 0; q < 6; q++)
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        int at;
-        bool in_;
-        (void)elem(q, h, cls, at, in_);
-        if (cls == 0) {
-          cube[q][at] = vals[0][0][q][h];
-          if constexpr (HJ) cubeJ[q][at] = vals[HJ ? 1 : 0][0][q][h];
-        } else {
-          cube[q][at] += vals[0][cls][q][h];
-          if constexpr (HJ) cubeJ[q][at] += vals[HJ ? 1 : 0][cls][q][h];
-        }
-      }
-    __syncthreads();
-  }
-
-  double e_acc = 0.0, ect_acc = 0.0;
-  triple_points<HJ, CZ, CSZ, true>(B1, B2, B3, no, distinct, epsabc, alias,
-                                   cube, &cubeJ[0][0], tA, tB, tC, sE, sV,
-                                   tid, e_acc, ect_acc);
-  e_acc = wave_sum(e_acc);
-  ect_acc = wave_sum(ect_acc);
-  const int wave = tid >> 6, lane = tid & 63;
-  if (lane == 0) {
-    red[0][wave] = e_acc;
-    red[1][wave] = ect_acc;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    const double e = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
-    const double ect = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
-    P.e_part[2 * gid] = e;
-    P.e_part[2 * gid + 1] = HJ ? ect : e;
-  }
+  if (nc == 6)
+    fused_orbit<GT, HJ, 6, CZ, CSZ>(P, gid, gslot, tid, no, a, b, c, B1, B2,
+                                    B3, distinct, epsabc, qc, alias, cube,
+                                    cubeJ, tA, tB, tC, sE, sV, red);
+  else if (nc == 3)
+    fused_orbit<GT, HJ, 3, CZ, CSZ>(P, gid, gslot, tid, no, a, b, c, B1, B2,
+                                    B3, distinct, epsabc, qc, alias, cube,
+                                    cubeJ, tA, tB, tC, sE, sV, red);
+  else
+    fused_orbit<GT, HJ, 1, CZ, CSZ>(P, gid, gslot, tid, no, a, b, c, B1, B2,
+                                    B3, distinct, epsabc, qc, alias, cube,
+                                    cubeJ, tA, tB, tC, sE, sV, red);
 }
 
 // Energy of each tuple slot: its triples' partials added in triple order.
