@@ -245,6 +245,7 @@ hipError_t launch_doubles_f64(const DevSystem &s, const BatchArgs &b,
   P.per_xcd = (P.total + 7) / 8;
   const int64_t grid = P.per_xcd * 8;
   if (grid <= 0) return hipSuccess;
+  if (grid > MAX_GRID) return hipErrorInvalidConfiguration;
 #define LAUNCH(A, B)                                                       \
   if (FM == A && FN == B)                                                  \
     hipLaunchKernelGGL((doubles_f64_kernel<A, B>), dim3((unsigned)grid),   \

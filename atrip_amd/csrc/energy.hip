@@ -797,6 +797,7 @@ hipError_t launch_wsum(const DevSystem &s, const BatchArgs &b, const void *A,
   const int nbk = (int)w_blocks(s.No), nbr = (nbk + 1) / 2;
   const int64_t grid = b.slots * (int64_t)nbr * nbr * nbr;
   if (grid <= 0) return hipSuccess;
+  if (grid > MAX_GRID) return hipErrorInvalidConfiguration;
   const int64_t NoP = g_pitch(s.No, g32), W3g = g_elems(s.No, g32);
   if (g32)
     hipLaunchKernelGGL(wsum_kernel<float>, dim3((unsigned)grid), dim3(256), 0,
@@ -839,6 +840,9 @@ hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
     P.tai_sign = tai_sign;
     const int64_t grid = (b.slots + 7) / 8 * 8 * ntri;
     if (grid <= 0) return hipSuccess;
+    // (the kernel decodes its block id and class-buffer offsets in 32 bits)
+    if (grid > MAX_GRID || g_elems(s.No, g32) >= (int64_t(1) << 31))
+      return hipErrorInvalidConfiguration;
     if (g32 && GJ)
       hipLaunchKernelGGL((energy_fused_kernel<float, true>), dim3((unsigned)grid),
                          dim3(256), 0, st, P);

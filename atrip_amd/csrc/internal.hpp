@@ -65,6 +65,9 @@ __host__ __device__ inline int64_t g_pitch(int64_t No, bool g32) {
   const int64_t q = g32 ? 16 : 8;
   return (No + q - 1) / q * q;
 }
+// Largest one-dimensional grid a launch takes (workgroups); launchers
+// refuse larger ones instead of truncating the count.
+constexpr int64_t MAX_GRID = 0xFFFFFFFFLL;
 // Elements of one tuple's class buffer.
 __host__ __device__ inline int64_t g_elems(int64_t No, bool g32) {
   return No * No * g_pitch(No, g32);
