@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "doubles_common.hpp"
@@ -792,6 +793,12 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
     }
   } guard;
   Sizes z;
+  // per-window sums of a window-sum sink, for every selected window
+  constexpr bool kWinSums = std::decay_t<Sink>::window_sums();
+  const size_t wsz = kWinSums ? (size_t)(2 * nsel) * sizeof(double) : 0;
+  double *wsum_dev = nullptr;
+  if (fixed && kWinSums)
+    throw Fail{"per-window sums are not an allocation-free run"};
   if (fixed) {
     while (bw > 1 && total(sizes(bw)) > fixed->bytes) bw--;
     z = sizes(bw);
@@ -826,8 +833,9 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
           "upload windows");
   } else {
     z = sizes(bw);
-    check(hipMalloc(&d_sel, z.windows), "alloc windows");
+    check(hipMalloc(&d_sel, a256(z.windows) + wsz), "alloc windows");
     guard.p = d_sel;
+    if (kWinSums) wsum_dev = (double *)((char *)d_sel + a256(z.windows));
     check(hipMemcpy(d_sel, sel.data(), z.windows, hipMemcpyHostToDevice),
           "upload windows");
     ensure((void *&)ctx->G, ctx->G_bytes, z.g, "alloc G");
@@ -932,7 +940,11 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
     check(launch_batch_reduce(s, b, (double *)e_slot, (double *)partials, bi,
                               st),
           "reduce kernel");
-    if (sink.active()) {
+    if constexpr (kWinSums) {
+      check(launch_window_sums((const double *)e_slot, b.nw, nb3,
+                               wsum_dev + 2 * w0, st),
+            "window sums kernel");
+    } else if (sink.active()) {
       host_slots.resize(2 * b.slots);
       check(hipMemcpyAsync(host_slots.data(), (double *)e_slot,
                            host_slots.size() * sizeof(double),
@@ -946,7 +958,13 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   check(hipMemcpyAsync(part.data(), (double *)partials, part.size() * sizeof(double),
                        hipMemcpyDeviceToHost, st),
         "download partials");
+  std::vector<double> wsums(kWinSums ? 2 * (size_t)nsel : 0);
+  if (kWinSums)
+    check(hipMemcpyAsync(wsums.data(), wsum_dev, wsz, hipMemcpyDeviceToHost,
+                         st),
+          "download window sums");
   check(hipStreamSynchronize(st), "run");
+  if constexpr (kWinSums) sink.windows(wsums);
   // Neumaier over the batch partials in batch order
   double s0 = 0, c0 = 0, s1 = 0, c1 = 0;
   auto add = [](double &sum, double &comp, double x) {
@@ -978,6 +996,7 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
 }
 
 struct NoSink {
+  static constexpr bool window_sums() { return false; }
   bool active() const { return false; }
   void operator()(const int32_t *, int, int, const std::vector<double> &) {}
 };
@@ -997,23 +1016,13 @@ void atrip_amd::run_plan_windows(atrip_hip_ctx *ctx, int sys, int precision,
                                  int nb, uint64_t wbegin, uint64_t wend,
                                  std::vector<double> &out) {
   out.clear();
+  // per-window (sum, comp) pairs formed on the device, downloaded once
   struct WinSink {
+    static constexpr bool window_sums() { return true; }
     std::vector<double> *out;
     bool active() const { return true; }
-    void operator()(const int32_t *, int nw, int nb,
-                    const std::vector<double> &slots) {
-      const int nb3 = nb * nb * nb;
-      for (int w = 0; w < nw; w++) {
-        double sum = 0.0, comp = 0.0;
-        for (int q = 0; q < nb3; q++) {
-          const double x = slots[2 * ((size_t)w * nb3 + q)];
-          const double t = sum + x;
-          comp += (std::fabs(sum) >= std::fabs(x)) ? (sum - t) + x : (x - t) + sum;
-          sum = t;
-        }
-        out->push_back(sum);
-        out->push_back(comp);
-      }
+    void windows(const std::vector<double> &pairs) {
+      out->insert(out->end(), pairs.begin(), pairs.end());
     }
   } sink{&out};
   WindowSel ws;
@@ -1283,6 +1292,7 @@ int atrip_hip_tuple_energies(atrip_hip_ctx *ctx, int sys, int precision,
     const uint64_t last = first + count;
     HostIndex hidx((uint64_t)s.Nv);
     struct Sink {
+      static constexpr bool window_sums() { return false; }
       bool active() const { return true; }
       const HostIndex *h;
       uint64_t first, last, Nv;

@@ -655,7 +655,7 @@ __global__ __launch_bounds__(256) void energy_fused_kernel(FusedParams P) {
   const int64_t a = P.windows[3 * w] * (int64_t)nb + ta;
   const int64_t b = P.windows[3 * w + 1] * (int64_t)nb + tb;
   const int64_t c = P.windows[3 * w + 2] * (int64_t)nb + tc;
-  const int64_t No = P.No, Nv = P.Nv, N2 = P.N2;
+  const int64_t No = P.No, Nv = P.Nv;
   const int tid = threadIdx.x;
   bool ok = tuple_ok(Nv, a, b, c);
   if (ok && (P.first > 0 || P.last != ~0ULL)) {
@@ -746,6 +746,24 @@ __global__ __launch_bounds__(256) void triple_sum_kernel(const double *e_part,
   }
   e_slot[2 * slot] = e;
   e_slot[2 * slot + 1] = ect;
+}
+
+// Per window of a batch, a compensated (Neumaier) sum of its slots' energies
+// in slot order: (sum, comp) to out[2 w], out[2 w + 1].
+__global__ __launch_bounds__(256) void window_sum_kernel(const double *e_slot,
+                                                         int nw, int64_t nb3,
+                                                         double *out) {
+  const int w = blockIdx.x * 256 + threadIdx.x;
+  if (w >= nw) return;
+  double sum = 0.0, comp = 0.0;
+  for (int64_t q = 0; q < nb3; q++) {
+    const double x = e_slot[2 * (w * nb3 + q)];
+    const double t = sum + x;
+    comp += (fabs(sum) >= fabs(x)) ? (sum - t) + x : (x - t) + sum;
+    sum = t;
+  }
+  out[2 * w] = sum;
+  out[2 * w + 1] = comp;
 }
 
 // Compensated sum of the slot energies of one batch: 256 threads each sum a
@@ -893,6 +911,14 @@ hipError_t launch_energy(const DevSystem &s, const BatchArgs &b,
     hipLaunchKernelGGL(energy_kernel<true>, grid, block, 0, st, P);
   else
     hipLaunchKernelGGL(energy_kernel<false>, grid, block, 0, st, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_window_sums(const double *e_slot, int nw, int64_t nb3,
+                              double *out, hipStream_t st) {
+  if (nw <= 0) return hipSuccess;
+  hipLaunchKernelGGL(window_sum_kernel, dim3((unsigned)((nw + 255) / 256)),
+                     dim3(256), 0, st, e_slot, nw, nb3, out);
   return hipGetLastError();
 }
 

@@ -309,6 +309,11 @@ hipError_t launch_energy_z(const DevSystem &s, const BatchArgs &b,
                            double tai_sign, double *e_slot, double *wcube,
                            hipStream_t st);
 
+// Per window of a batch (nw windows of nb3 slots), the Neumaier sum of its
+// slot energies in slot order: out[2 w] = sum, out[2 w + 1] = compensation.
+hipError_t launch_window_sums(const double *e_slot, int nw, int64_t nb3,
+                              double *out, hipStream_t st);
+
 // Deterministic compensated sum of the valid slot energies of a batch into
 // partials[4*index + {sum, comp, sum_ct, comp_ct}].
 hipError_t launch_batch_reduce(const DevSystem &s, const BatchArgs &b,
