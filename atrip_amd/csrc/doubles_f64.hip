@@ -141,8 +141,7 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
 
   // Software pipeline: two operand sets in ping-pong, so the loads of one
   // chunk are in flight while the MFMAs of the other run (no register
-  // copies between the sets, hence no wait on the in-flight loads).  The
-  // four K segments run one after the other.
+  // copies between the sets, hence no wait on the in-flight loads).
   const double *ap[FM], *bp[FN];
 #define LOAD(AV, BV, KOFF)                                              \
   {                                                                     \
@@ -167,24 +166,51 @@ __global__ __launch_bounds__(256) void doubles_f64_kernel(DoublesParams P) {
     if (!(HALF)) { STEPS(AV, BV, 2, 4) }                                \
   }
   d4 a0[FM], b0[FN], a1[FM], b1[FN];
-#pragma unroll 1
-  for (int sg = 0; sg < 4; sg++) {
-    SET_PTRS(sg);
-    // the prefetch is issued unconditionally (clamped to the last chunk) so
-    // every path into a COMPUTE has the same loads outstanding
-    const int64_t last = (sg < 2 ? Kv : Ko) - KCHUNK;
-    // the segment's last chunk holds at most 8 occupied kappa
-    const bool tail8 = (sg < 2 ? P.Nv : No) - last <= 8;
-    LOAD(a0, b0, 0);
-    for (int64_t k0 = 0;; k0 += 2 * KCHUNK) {
-      LOAD(a1, b1, k0 + KCHUNK < last ? k0 + KCHUNK : last);
-      COMPUTE(a0, b0, tail8 && k0 == last);
-      if (k0 + KCHUNK > last) break;
-      LOAD(a0, b0, k0 + 2 * KCHUNK < last ? k0 + 2 * KCHUNK : last);
-      COMPUTE(a1, b1, tail8 && k0 + KCHUNK == last);
-      if (k0 + 2 * KCHUNK > last) break;
-    }
+  // One pipeline through the four segments: (sg, c) is the next chunk to
+  // load, and past a segment's end the pointers move to the next segment's
+  // chunk 0 in place, so every segment but the first starts with its first
+  // chunk in flight (3 % faster at No=40 than a pipeline restarted per
+  // segment).  The prefetch is issued unconditionally (after the last chunk
+  // it re-reads it, a load never used) so every path into a COMPUTE has the
+  // same loads outstanding.  h0 / h1: the chunk in set 0 / 1 is a segment's
+  // last and holds at most 8 occupied kappa.
+  int sg = 0;
+  int64_t nch = Kv / KCHUNK, c = 0;
+  bool t8 = P.Nv - (Kv - KCHUNK) <= 8;  // tail rule of segment sg
+  SET_PTRS(0);
+#define ADVANCE()                                                            \
+  {                                                                          \
+    if (++c == nch && ++sg < 4) {                                            \
+      SET_PTRS(sg);                                                          \
+      const int64_t K_ = sg < 2 ? Kv : Ko;                                   \
+      nch = K_ / KCHUNK;                                                     \
+      t8 = (sg < 2 ? P.Nv : No) - (K_ - KCHUNK) <= 8;                        \
+      c = 0;                                                                 \
+    }                                                                        \
   }
+#define NEXT_OFF() (KCHUNK * (sg < 4 ? c : nch - 1))
+#define HALF_NEXT() (sg < 4 && t8 && c == nch - 1)
+  bool h0 = HALF_NEXT(), h1 = false;
+  LOAD(a0, b0, 0);
+  ADVANCE();
+#pragma unroll 1
+  for (;;) {
+    const bool v1 = sg < 4;
+    h1 = HALF_NEXT();
+    LOAD(a1, b1, NEXT_OFF());
+    if (v1) ADVANCE();
+    COMPUTE(a0, b0, h0);
+    if (!v1) break;
+    const bool v2 = sg < 4;
+    h0 = HALF_NEXT();
+    LOAD(a0, b0, NEXT_OFF());
+    if (v2) ADVANCE();
+    COMPUTE(a1, b1, h1);
+    if (!v2) break;
+  }
+#undef ADVANCE
+#undef NEXT_OFF
+#undef HALF_NEXT
 #undef LOAD
 #undef STEPS
 #undef COMPUTE

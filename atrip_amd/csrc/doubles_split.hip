@@ -176,25 +176,42 @@ __global__ __launch_bounds__(256) void doubles_split_kernel(SplitParams P) {
       for (int r = 0; r < 16; r++) acc[f][g][r] = 0.f;
 
   // two operand sets in ping-pong: the loads of chunk c + 1 are in flight
-  // behind the MFMAs of chunk c
+  // behind the MFMAs of chunk c.  One pipeline runs through the four
+  // segments: (sg, c) is the next chunk to load, and past a segment's end
+  // the pointers move to the next segment's chunk 0 in place, so every
+  // segment but the first starts with its first chunk in flight.  After the
+  // last chunk the prefetch re-reads it (a load never used).
   static_assert(6 * (FM + FN) <= 63, "two chunks of loads must fit vmcnt");
   SParts a0[FM], b0[FN], a1[FM], b1[FN];
-#define CLAMP(K_) ((K_) < last ? (K_) : last)
-#pragma unroll 1
-  for (int sg = 0; sg < 4; sg++) {
-    SET_PTRS(sg);
-    const int64_t last = (sg < 2 ? Kv : Ko) - KCHUNK;
-    LOAD(a0, b0, 0);
-    for (int64_t k0 = 0;; k0 += 2 * KCHUNK) {
-      LOAD(a1, b1, CLAMP(k0 + KCHUNK));
-      COMPUTE(a0, b0);
-      if (k0 + KCHUNK > last) break;
-      LOAD(a0, b0, CLAMP(k0 + 2 * KCHUNK));
-      COMPUTE(a1, b1);
-      if (k0 + 2 * KCHUNK > last) break;
-    }
+  int sg = 0;
+  int64_t nch = Kv / KCHUNK, c = 0;
+  SET_PTRS(0);
+#define ADVANCE()                                                            \
+  {                                                                          \
+    if (++c == nch && ++sg < 4) {                                            \
+      SET_PTRS(sg);                                                          \
+      nch = (sg < 2 ? Kv : Ko) / KCHUNK;                                     \
+      c = 0;                                                                 \
+    }                                                                        \
   }
-#undef CLAMP
+#define NEXT_OFF() (KCHUNK * (sg < 4 ? c : nch - 1))
+  LOAD(a0, b0, 0);
+  ADVANCE();
+#pragma unroll 1
+  for (;;) {
+    const bool v1 = sg < 4;
+    LOAD(a1, b1, NEXT_OFF());
+    if (v1) ADVANCE();
+    COMPUTE(a0, b0);
+    if (!v1) break;
+    const bool v2 = sg < 4;
+    LOAD(a0, b0, NEXT_OFF());
+    if (v2) ADVANCE();
+    COMPUTE(a1, b1);
+    if (!v2) break;
+  }
+#undef ADVANCE
+#undef NEXT_OFF
 #undef SET_PTRS
 #undef LOAD
 #undef COMPUTE
