@@ -149,12 +149,13 @@ __host__ __device__ inline int t32_fn(int t) {
 }
 // Rate of each variant relative to 4x4, per useful tile element: variants
 // 4-9 measured in bf16x3 on MI355X (mean of No=40, Nv=400 and No=60,
-// Nv=600, each divided by its padded-grid fraction); variants 0-3 from the
-// operand bytes per MFMA cycle, FM FN / (FM + FN), scaled by the ratio that
-// measured variants 4 and 5 show over the same formula.
+// Nv=600, each divided by its padded-grid fraction; 2x5 re-measured against
+// 3x5 at No=40 with the one-pipeline K loop, 1.5 % faster there); variants
+// 0-3 from the operand bytes per MFMA cycle, FM FN / (FM + FN), scaled by the
+// ratio that measured variants 4 and 5 show over the same formula.
 inline double t32_rate(int t) {
   const double v[N_T32_TILES] = {0.62, 0.75, 0.75, 0.83, 0.985,
-                                 0.896, 0.988, 1.0, 0.989, 0.952};
+                                 0.966, 0.988, 1.0, 0.989, 0.952};
   return v[t];
 }
 // Automatic variant: rate times the useful fraction of the padded
