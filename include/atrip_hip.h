@@ -211,7 +211,16 @@ int atrip_hip_tuple_energies(atrip_hip_ctx *ctx, int sys, int precision,
  *             edge 6 or 12 is 15 % better)
  *   "batch"   windows per launch
  *   "timing"  1: time every hot-kernel launch with HIP events
- *   "tile"    doubles-kernel tile variant 0..7, -1 (default) = automatic
+ *   "tile"    doubles-kernel tile variant: 0..7 for the FP64 kernel, 0..9
+ *             for the FP32-product kernels (a variant a kernel lacks
+ *             means automatic for it), -1 (default) = automatic
+ *   "epilogue"  0 (default): fused class sums + energy per block triple;
+ *               1: class sums written as W cubes first (complex systems
+ *               always take this form)
+ *   "serve_plain_no"  servers run requests with No at or above it through
+ *               the plain class-GEMM path and the rest in the persistent
+ *               batched launch (default 16; a bf16x3 server runs every
+ *               request on the plain path)
  *   "stage_kb"  device staging buffer of host uploads, KiB (default 4 GiB;
  *               at least one slab of the slowest CTF index is staged per
  *               step whatever the budget), also the bound of the pinned
