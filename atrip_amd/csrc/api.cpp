@@ -786,12 +786,6 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
   int bw = std::max(1, ctx->batch);
   void *Gbuf = nullptr, *e_slot = nullptr, *wcube = nullptr,
        *partials = nullptr, *d_sel = nullptr;
-  struct FreeOnExit {
-    void *p = nullptr;
-    ~FreeOnExit() {
-      if (p) (void)hipFree(p);
-    }
-  } guard;
   Sizes z;
   // per-window sums of a window-sum sink, for every selected window
   constexpr bool kWinSums = std::decay_t<Sink>::window_sums();
@@ -833,10 +827,15 @@ RunResult run_range(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
           "upload windows");
   } else {
     z = sizes(bw);
-    check(hipMalloc(&d_sel, a256(z.windows) + wsz), "alloc windows");
-    guard.p = d_sel;
+    // the window list lives in a context buffer grown on demand (no
+    // allocation and release per call); the copy is ordered on the stream
+    // before the kernels that read it, and `sel` outlives it (the run
+    // waits for the stream before returning)
+    ensure(ctx->sel, ctx->sel_bytes, a256(z.windows) + wsz, "alloc windows");
+    d_sel = ctx->sel;
     if (kWinSums) wsum_dev = (double *)((char *)d_sel + a256(z.windows));
-    check(hipMemcpy(d_sel, sel.data(), z.windows, hipMemcpyHostToDevice),
+    check(hipMemcpyAsync(d_sel, sel.data(), z.windows, hipMemcpyHostToDevice,
+                         ctx->stream),
           "upload windows");
     ensure((void *&)ctx->G, ctx->G_bytes, z.g, "alloc G");
     ensure((void *&)ctx->e_slot, ctx->e_slot_bytes, z.e_slot, "alloc e_slot");
@@ -1085,6 +1084,7 @@ int atrip_hip_destroy(atrip_hip_ctx *ctx) {
   if (ctx->G) (void)hipFree(ctx->G);
   if (ctx->e_slot) (void)hipFree(ctx->e_slot);
   if (ctx->partials) (void)hipFree(ctx->partials);
+  if (ctx->sel) (void)hipFree(ctx->sel);
   if (ctx->bscratch) (void)hipFree(ctx->bscratch);
   if (ctx->wcube) (void)hipFree(ctx->wcube);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

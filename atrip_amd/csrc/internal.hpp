@@ -234,6 +234,8 @@ struct atrip_hip_ctx {
   size_t e_slot_bytes = 0;
   double *partials = nullptr;  // per-batch (sum, comp) x 2
   size_t partials_bytes = 0;
+  void *sel = nullptr;  // a run's window list (and per-window sums)
+  size_t sel_bytes = 0;
   // batched form: tuned plans keyed by (No, padded Nv, precision),
   // workgroup scratch
   std::map<std::array<int64_t, 3>, atrip_amd::BatchTune> btune;
