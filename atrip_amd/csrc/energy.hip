@@ -450,7 +450,7 @@ void energy_kernel(EnergyParams P) {
 // e_part.  triple_sum_kernel adds a tuple's triples in order.  Nothing
 // about W leaves the chip: no cube is written to memory and read back.
 struct FusedParams {
-  const void *G, *GJ;  // class buffers [3][slots][No^3] (FP64 or FP32)
+  const void *G, *GJ;  // class buffers [3][slots][No^2 rows][NoP] (FP64 or FP32)
   const double *Tai, *VHH, *ei, *ea;
   const int32_t *windows;
   const uint64_t *aprefix;
