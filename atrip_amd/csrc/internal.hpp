@@ -226,7 +226,7 @@ struct atrip_hip_ctx {
   int serve_plain_no = 16;  // served requests with No >= this: plain path
   int64_t stage_kb = 4 << 20;  // device staging of host uploads (KiB)
   // scratch, grown on demand
-  double *G = nullptr;       // class buffers [ncls][slots][No^3]
+  double *G = nullptr;       // class buffers [ncls][slots][g_elems(No)]
   size_t G_bytes = 0;
   double *wcube = nullptr;   // W as cubes for the energy kernels
   size_t wcube_bytes = 0;
@@ -265,7 +265,7 @@ struct BatchArgs {
   int64_t slots;           // nw * nb^3
 };
 
-// Doubles contractions in FP64 MFMA: class buffers G[3][slots][No^3].
+// Doubles contractions in FP64 MFMA: class buffers G[3][slots][g_elems(No)].
 hipError_t launch_doubles_f64(const DevSystem &s, const BatchArgs &b,
                               bool use_j, double *G, int tile,
                               hipStream_t st);
