@@ -68,7 +68,12 @@ __device__ inline void class_out(int cl, int64_t p, int64_t q, int64_t rr,
 }
 
 // Tile shapes (wave tile FM x FN fragments of 16x16, 2x2 waves) and the
-// fraction of peak each reached on MI355X at No=40, Nv=400 in FP64.
+// fraction of peak each reaches per useful tile element on MI355X in FP64:
+// 5x5 at No=40, Nv=400; 4x2, 4x4, 3x3, 4x5 and 5x4 relative to it from
+// whole-system runs at No=26/Nv=308, 31/270, 38/204 and 20/183, each divided
+// by its padded-grid fraction (5x4 measured 4 % above 5x5 there, entered
+// equal to it, since no shape measured both at full use); 2x2 and 2x4 from
+// an earlier kernel at No=40.
 constexpr int N_TILES = 8;
 __host__ __device__ inline int tile_fm(int t) {
   const int v[N_TILES] = {2, 4, 2, 4, 3, 5, 4, 5};
@@ -79,7 +84,7 @@ __host__ __device__ inline int tile_fn(int t) {
   return v[t];
 }
 inline double tile_rate(int t) {
-  const double v[N_TILES] = {0.44, 0.56, 0.51, 0.57, 0.53, 0.69, 0.60, 0.60};
+  const double v[N_TILES] = {0.44, 0.575, 0.51, 0.627, 0.61, 0.69, 0.677, 0.69};
   return v[t];
 }
 
