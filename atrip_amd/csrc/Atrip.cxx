@@ -203,6 +203,21 @@ Atrip::Output run_impl(Atrip::Input<F> const &in, size_t np) {
                           : std::max<uint64_t>(1, (last - first) / 100);
   if (seg == 0) seg = 1;
 
+  // Progress lines at the reference's reporting points (:402-404, :734):
+  // iteration numbers (tuple index + 1) that are multiples of iteration_mod,
+  // or of percentage_mod % of the run when that is set, and the 1 % point.
+  // An iteration_mod of -1 (the default) converts to SIZE_MAX as there, so
+  // only the 1 % point reports.  Here the line comes at the end of the
+  // segment that reaches such a point and names the segment's last iteration.
+  const uint64_t report_mod =
+      in.percentage_mod > 0
+          ? (uint64_t)((double)n * in.percentage_mod / 100.0)
+          : (uint64_t)(size_t)in.iteration_mod;
+  const uint64_t one_percent = (uint64_t)((double)n * 0.01);
+  // the reference counts complex products as four real ones (bench/main.cxx:119)
+  const double gflop_tuple = 12.0 * (double)No * No * No * (double)(No + Nv) /
+                             1e9 * (std::is_same<F, Complex>::value ? 4.0 : 1.0);
+
   const auto t0 = std::chrono::steady_clock::now();
   double raw = 0.0, raw_ct = 0.0;  // sums before the final sign
   for (uint64_t s0 = first; s0 < last; s0 += seg) {
@@ -230,6 +245,16 @@ Atrip::Output run_impl(Atrip::Input<F> const &in, size_t np) {
     }
     if (IterationDescription::descriptor)
       IterationDescription::descriptor({s0 + cnt, n, elapsed});
+    const uint64_t it0 = s0 + 1, it1 = s0 + cnt;  // iterations of the segment
+    if (report_mod == 0 || it1 / report_mod > (it0 - 1) / report_mod ||
+        (one_percent >= it0 && one_percent <= it1)) {
+      std::cout << "Atrip: iteration " << it1 << " [" << 100 * it1 / n
+                << "%] ("
+                << (elapsed > 0 ? gflop_tuple * (double)(it1 - first) / elapsed
+                                : -1.0)
+                << "GF)" << std::endl;
+      if (in.chrono) std::cout << "  iterations :: " << elapsed << std::endl;
+    }
   }
   const double elapsed =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
@@ -244,9 +269,6 @@ Atrip::Output run_impl(Atrip::Input<F> const &in, size_t np) {
             << out.energy << std::endl;
   std::cout << "Atrip: Energy (cT): " << std::setprecision(15) << std::setw(23)
             << out.ct_energy << std::endl;
-  // the reference counts complex products as four real ones (bench/main.cxx:119)
-  const double gflop_tuple = 12.0 * (double)No * No * No * (double)(No + Nv) /
-                             1e9 * (std::is_same<F, Complex>::value ? 4.0 : 1.0);
   if (elapsed > 0)
     std::cout << "atrip:flops(iterations): "
               << gflop_tuple * (double)(last - first) / elapsed << "\n";

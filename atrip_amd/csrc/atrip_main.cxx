@@ -18,11 +18,13 @@ namespace {
 
 void usage() {
   std::cerr
-      << "usage: atrip --no N --nv N [--seed S] [--max-iterations K]\n"
+      << "usage: atrip --no N --nv N [--dist group|naive] [--seed S]\n"
+         "             [--max-iterations K] [--mod K] [-% P]\n"
          "             [--checkpoint-path P] [--nocheckpoint]\n"
          "             [--checkpoint-it K] [--checkpoint-% P] [--ijkabc]\n"
          "             [--cT] [--mixed] [--mixed-form bf16x3|split_fp32|fp32]\n"
-         "             [--complex] [--device D] [--chrono]\n"
+         "             [--complex] [--device D] [--nochrono] [--keep-vppph]\n"
+         "             [--rank-round-robin] [--barrier] [--blocking]\n"
          "             [--ei P] [--ea P] [--Tph P] [--Tpphh P] [--Vpphh P]\n"
          "             [--Vhhhp P] [--Vppph P] [--Jppph P] [--Jhhhp P]\n";
 }
@@ -136,14 +138,17 @@ CTF::Tensor<F> *read_or_make_j(int order, std::vector<int> lens, int tensor,
   return t;
 }
 
+// Defaults of the reference driver's options (bench/main.cxx:410-470).
 struct Options {
-  int no = 0, nv = 0, device = 0;
+  int no = 0, nv = 0, device = 0, it_mod = -1, pct_mod = 10;
   uint64_t seed = 1;
   size_t max_it = 0, ck_it = 0;
-  float ck_pct = 10;
+  float ck_pct = 0;
   bool nockpt = false, ijkabc = false, ct = false, mixed = false,
-       chrono = false;
-  std::string ck_path = "atrip-checkpoint.yaml";
+       chrono = true, keep_vppph = false, rank_round_robin = false,
+       barrier = false, blocking = false;
+  bool dist_naive = false;  // --dist naive, else group (GROUP_AND_SORT)
+  std::string ck_path = "checkpoint.yaml";
   std::string mixed_form = "bf16x3";
   std::string ei, ea, tph, tpphh, vpphh, vhhhp, vppph, jppph, jhhhp;
 };
@@ -192,10 +197,18 @@ int run(const Options &o) try {
                 .with_writeCheckpoint(!o.nockpt)
                 .with_ijkabc(o.ijkabc)
                 .with_chrono(o.chrono)
+                .with_iteration_mod(o.it_mod)
+                .with_percentage_mod(o.pct_mod)
+                .with_tuples_distribution(
+                    o.dist_naive ? Atrip::Input<F>::NAIVE
+                                 : Atrip::Input<F>::GROUP_AND_SORT)
+                .with_rank_round_robin(o.rank_round_robin)
+                .with_barrier(o.barrier)
+                .with_blocking(o.blocking)
                 .with_device(o.device)
                 .with_mixed_precision(o.mixed)
                 .with_mixed_form(o.mixed_form)
-                .with_delete_Vppph(true);
+                .with_delete_Vppph(!o.keep_vppph);
   const auto out = Atrip::run(in);
   std::printf("Energy: %.15g\nEnergy (cT): %.15g\n", out.energy,
               out.ct_energy);
@@ -240,6 +253,22 @@ int main(int argc, char **argv) {
     else if (a == "--complex") cplx = true;
     else if (a == "--device") o.device = std::atoi(next());
     else if (a == "--chrono") o.chrono = true;
+    else if (a == "--nochrono") o.chrono = false;
+    else if (a == "--keep-vppph") o.keep_vppph = true;
+    else if (a == "--rank-round-robin") o.rank_round_robin = true;
+    else if (a == "--barrier") o.barrier = true;
+    else if (a == "--blocking") o.blocking = true;
+    else if (a == "--mod") o.it_mod = std::atoi(next());
+    else if (a == "-%") o.pct_mod = std::atoi(next());
+    else if (a == "--dist") {
+      const std::string d = next();
+      if (d == "group") o.dist_naive = false;
+      else if (d == "naive") o.dist_naive = true;
+      else {
+        std::cerr << "ERROR: --dist: '" << d << "' is not group or naive\n";
+        return 2;
+      }
+    }
     else if (a == "--ei") o.ei = next();
     else if (a == "--ea") o.ea = next();
     else if (a == "--Tph") o.tph = next();

@@ -185,3 +185,71 @@ def test_cli_tensor_file_errors(tmp_path):
                         str(tmp_path / "j2.bin")], capture_output=True, text=True,
                        timeout=120, cwd=tmp_path)
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+# Every option of the reference driver (bench/main.cxx:410-470), with values.
+REF_DRIVER_FLAGS = ["--dist", "group", "--mod", "50", "-%", "10", "--keep-vppph",
+                    "--nochrono", "--rank-round-robin", "--barrier", "--blocking",
+                    "--checkpoint-it", "0", "--checkpoint-%", "0",
+                    "--checkpoint-path", "ck.yaml", "--nocheckpoint"]
+
+
+def test_cli_takes_the_reference_driver_options(tmp_path):
+    """A command line written for the reference driver parses: without a GPU it
+    ends at device creation (exit 1, an ERROR line), never at the option
+    parser (exit 2, usage); with one it runs."""
+    _need(CLI)
+    r = subprocess.run([CLI, "--no", "3", "--nv", "5"] + REF_DRIVER_FLAGS,
+                       capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert r.returncode in (0, 1), r.stdout + r.stderr
+    assert "usage" not in r.stderr
+    if r.returncode == 1:
+        assert "ERROR:" in r.stderr
+    r = subprocess.run([CLI, "--no", "3", "--nv", "5", "--dist", "round"],
+                       capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 2 and "--dist" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_reference_options_and_progress(oracle, tmp_path):
+    """The reference driver's options run to the oracle's energy, and progress
+    lines come at its reporting points: every -% 10 % of the run (iterations
+    that are multiples of n/10) and the 1 % point (Atrip.cxx:402-404, :734),
+    each at the end of the segment that reaches it."""
+    _need(CLI)
+    no, nv, seed = 4, 10, 2
+    n = nv * (nv + 1) * (nv + 2) // 6 - nv          # 210 tuples
+    r = subprocess.run([CLI, "--no", str(no), "--nv", str(nv), "--seed", str(seed)]
+                       + REF_DRIVER_FLAGS, capture_output=True, text=True,
+                       timeout=300, cwd=tmp_path)
+    e, _ = _energies(r)
+    e_ref, _ = oracle.run(no, nv, oracle.synthetic(no, nv, seed))
+    assert abs(e - e_ref) <= 1e-10 * abs(e_ref)
+    its = [int(m) for m in re.findall(r"^Atrip: iteration (\d+) \[", r.stdout, re.M)]
+    # segments of 1 % of the run (2 tuples): the point n // 100 = 2 and each
+    # multiple of n // 10 = 21 land in exactly one segment each
+    mod = n * 10 // 100
+    want = sorted({2} | {((k * mod + 1) // 2) * 2 for k in range(1, n // mod + 1)})
+    assert its == want, (its, want)
+    assert not (tmp_path / "ck.yaml").exists()      # --nocheckpoint: no writes
+
+
+@pytest.mark.gpu
+def test_cli_default_checkpoint_resumes(oracle, tmp_path):
+    """With the reference driver's defaults (checkpoint interval 0 %, path
+    checkpoint.yaml) a run leaves its checkpoint behind and the same command
+    run again resumes from it to the same energy."""
+    _need(CLI)
+    no, nv, seed = 4, 9, 5
+    cmd = [CLI, "--no", str(no), "--nv", str(nv), "--seed", str(seed)]
+    r1 = subprocess.run(cmd, capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    e1, _ = _energies(r1)
+    ck = tmp_path / "checkpoint.yaml"
+    assert ck.exists()
+    assert "Reading checkpoint" not in r1.stdout
+    r2 = subprocess.run(cmd, capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    e2, _ = _energies(r2)
+    assert "Reading checkpoint" in r2.stdout
+    e_ref, _ = oracle.run(no, nv, oracle.synthetic(no, nv, seed))
+    assert abs(e1 - e_ref) <= 1e-10 * abs(e_ref)
+    assert abs(e2 - e_ref) <= 1e-10 * abs(e_ref)
