@@ -1046,6 +1046,28 @@ void atrip_amd::run_fixed(atrip_hip_ctx *ctx, int sys, int precision,
 
 extern "C" {
 
+int atrip_hip_plan_model(int64_t No, int64_t Nv, int precision, int window,
+                         int *nb, int *fm, int *fn) {
+  if (No < 1 || Nv < 1 || window < 0 || window > 32) return 1;
+  if (precision < ATRIP_FP64 || precision > ATRIP_BF16X3) return 1;
+  // the same choices as window_edge and the doubles launchers make
+  const int e = window > 0 ? window : auto_window(No, Nv);
+  int m, n;
+  if (precision == ATRIP_FP64) {
+    const int t = pick_tile(-1, No, e);
+    m = tile_fm(t);
+    n = tile_fn(t);
+  } else {
+    const int t = pick_t32_tile(-1, No, e);
+    m = t32_fm(t);
+    n = t32_fn(t);
+  }
+  if (nb) *nb = e;
+  if (fm) *fm = m;
+  if (fn) *fn = n;
+  return 0;
+}
+
 int atrip_hip_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;

@@ -36,7 +36,7 @@ SYMBOLS = (
     "atrip_hip_serve_set_policy", "atrip_hip_serve_clock_ms",
     "atrip_hip_serve_submit_synthetic", "atrip_hip_serve_submit",
     "atrip_hip_serve_step", "atrip_hip_serve_result", "atrip_hip_serve_stats",
-    "atrip_hip_run_noalloc",
+    "atrip_hip_run_noalloc", "atrip_hip_plan_model",
 )
 
 TENSORS = ("ei", "ea", "Tai", "Tabij", "Vabij", "Vijka", "Vabci", "Jijka", "Jabci")
@@ -122,6 +122,8 @@ def load():
                                        C.POINTER(C.c_double)]
     L.atrip_hip_batch_model.argtypes = [C.c_int64, C.c_int64, C.c_int,
                                         C.c_int64, C.c_int, ip, ip, ip]
+    L.atrip_hip_plan_model.argtypes = [C.c_int64, C.c_int64, C.c_int, C.c_int,
+                                       ip, ip, ip]
     dp = C.POINTER(C.c_double)
     u64p = C.POINTER(C.c_uint64)
     L.atrip_hip_serve_create.argtypes = [vp, C.c_uint64, C.c_int, C.c_double,
@@ -153,6 +155,18 @@ def batch_model(no: int, nv: int, scratch_mb: int = 8192, grid: int = 256,
                                       grid, C.byref(nb), C.byref(fm), C.byref(fn))
     if rc != 0:
         raise HipError(f"atrip_hip_batch_model({no}, {nv}) failed ({rc})")
+    return nb.value, fm.value, fn.value
+
+
+def plan_model(no: int, nv: int, precision="fp64", window: int = 0):
+    """Plan of the plain path (window edge, fm, fn) that atrip_hip_run takes
+    for this shape with the automatic tile; host only."""
+    nb, fm, fn = C.c_int(), C.c_int(), C.c_int()
+    rc = load().atrip_hip_plan_model(no, nv, PRECISIONS[precision], window,
+                                     C.byref(nb), C.byref(fm), C.byref(fn))
+    if rc != 0:
+        raise HipError(f"atrip_hip_plan_model({no}, {nv}, {precision}, {window}) "
+                       f"failed ({rc})")
     return nb.value, fm.value, fn.value
 
 

@@ -182,6 +182,17 @@ int atrip_hip_run_noalloc(atrip_hip_ctx *ctx, int sys, int precision,
                           int ijkabc, uint64_t first, uint64_t count,
                           double *energy, double *ct_energy);
 
+/* Plan that atrip_hip_run takes for a shape, on the host only (no device
+ * needed): *nb the window edge (`window` 0 = automatic, as the context
+ * option) and *fm x *fn the doubles kernel's wave tile in MFMA fragments
+ * (16x16 FP64 fragments for ATRIP_FP64, 32x32 for the other forms), the
+ * automatic pick of a context whose "tile" option is -1.  No counterpart in
+ * the reference, which has no tiles; for callers that size or schedule runs
+ * from the plan.  Returns nonzero for No or Nv below 1, a window outside
+ * 0..32 or an unknown precision. */
+int atrip_hip_plan_model(int64_t No, int64_t Nv, int precision, int window,
+                         int *nb, int *fm, int *fn);
+
 /* (T) energy of a systematic sample of the tuple space.  The window plan
  * cuts the virtual index into blocks of edge "window"; its windows are the
  * block triples (alpha <= beta <= gamma) holding a tuple, and a row is all
