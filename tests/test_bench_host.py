@@ -49,3 +49,12 @@ def test_sample_never_exceeds_the_space(monkeypatch):
     count, _, _ = bench._sized_sample(None, 2, 6, None, 1, 10.0)
     assert count <= bench.n_tuples(6)
     assert max(calls) <= bench.n_tuples(6)
+
+
+def test_more_than_one_gpu_is_refused(capsys, monkeypatch):
+    # one GPU only: --gpus above 1, or a launcher's WORLD_SIZE above 1, ends
+    # before any device work with a message that says so
+    assert bench.main(["--gpus", "2"]) == 2
+    assert "one GPU" in capsys.readouterr().err
+    monkeypatch.setenv("WORLD_SIZE", "8")
+    assert bench.main([]) == 2
