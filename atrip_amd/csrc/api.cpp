@@ -306,17 +306,25 @@ struct ReaderSource : Source {
     comp = comp_;
     half = std::max<int64_t>(comp, cap_bytes / (2 * (int64_t)sizeof(double)));
     half = half / comp * comp;
-    check(hipHostMalloc((void **)&pinned, (size_t)(2 * half) * sizeof(double),
-                        hipHostMallocDefault),
-          "alloc pinned staging");
-    for (auto &e : done) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+    try {
+      check(hipHostMalloc((void **)&pinned,
+                          (size_t)(2 * half) * sizeof(double),
+                          hipHostMallocDefault),
+            "alloc pinned staging");
+      for (auto &e : done)
+        check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+    } catch (...) {
+      release();  // no destructor runs for a constructor that throws
+      throw;
+    }
   }
-  ~ReaderSource() override {
+  ~ReaderSource() override { release(); }
+  void release() {
     for (int h = 0; h < 2; h++)
       if (recorded[h]) (void)hipEventSynchronize(done[h]);  // copies done
-    for (auto e : done)
-      if (e) (void)hipEventDestroy(e);
-    if (pinned) (void)hipHostFree(pinned);
+    for (auto &e : done)
+      if (e) (void)hipEventDestroy(e), e = nullptr;
+    if (pinned) (void)hipHostFree(pinned), pinned = nullptr;
   }
   void read(int t, int64_t first, int64_t count, double *out) {
     if (fn(user, t, first, count, out) != 0)

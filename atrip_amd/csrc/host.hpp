@@ -68,19 +68,18 @@ void run_fixed(atrip_hip_ctx *ctx, int sys, int precision, int ijkabc,
                uint64_t first, uint64_t count, void *base, size_t bytes,
                double *energy, double *ct_energy);
 
-// Windows of a system's plan at the context's window edge (lexicographic
-// block triples holding a tuple).
 // Window edge of the plain path for system s: the "window" option, or
 // when it is 0 the automatic edge of the system's shape.
 int window_edge(const atrip_hip_ctx *ctx, const DevSystem &s);
-// Windows of the plain-path plan of edge nb of a system.
+// Number of windows of the plain-path plan of edge nb of a system (the
+// lexicographic block triples holding a tuple).
 uint64_t plan_windows(atrip_hip_ctx *ctx, int sys, int nb);
 // Plain-path run (doubles kernel, energy epilogue) over windows
 // [wbegin, wend) of the plan of edge nb of a system, every tuple of each,
-// in launches of the context's "batch" windows.  out receives per window, in plan order, the
-// (sum, comp) of a compensated sum of its raw tuple energies in slot order,
-// so a window's pair does not depend on how windows were grouped into
-// launches or calls.
+// in launches of the context's "batch" windows.  out receives per window,
+// in plan order, the (sum, comp) of a compensated sum of its raw tuple
+// energies in slot order, so a window's pair does not depend on how windows
+// were grouped into launches or calls.
 void run_plan_windows(atrip_hip_ctx *ctx, int sys, int precision, int nb,
                       uint64_t wbegin, uint64_t wend,
                       std::vector<double> &out);
