@@ -92,6 +92,14 @@ void fail_request(atrip_hip_server *srv, Request &r, const std::string &why) {
   r.t_done = srv->now();
 }
 
+// Releases an admitted request's system on a failure path; a failure to
+// release is left for the next call to report.
+void drop_system(atrip_hip_server *srv, Request &r) {
+  if (r.sys) (void)atrip_hip_release(srv->ctx, r.sys);
+  r.sys = 0;
+  srv->resident -= r.bytes;
+}
+
 void retire(atrip_hip_server *srv, Request &r) {
   // src/atrip/Atrip.cxx:1108-1111: E = -sum
   r.energy = -(r.sum + r.comp);
@@ -162,24 +170,36 @@ void admit(atrip_hip_server *srv, int grid) {
     }
     srv->queue.pop_front();
     r.sys = sys;
-    r.state = ACTIVE;
     r.t_admit = srv->now();
     srv->resident += r.bytes;
     srv->peak_resident = std::max(srv->peak_resident, srv->resident);
     r.plain = runs_plain(srv, r.No);
-    if (r.plain) {
-      // the request keeps the window edge it was admitted with, whatever
-      // the context's option does later
-      r.plan.nb = window_edge(ctx, get_sys(ctx, sys));
-      r.nwin = plan_windows(ctx, sys, r.plan.nb);
-      r.plan.measured = false;
-      r.plan.sec_window = plain_window_estimate(r.No, r.Nv, r.plan.nb,
-                                                srv->precision, grid);
-    } else {
-      r.plan = plan_for(ctx, get_sys(ctx, sys), grid, srv->precision);
-      r.wins = windows_of(r.Nv, r.plan.nb);
-      r.nwin = r.wins.size();
+    try {
+      if (r.plain) {
+        // the request keeps the window edge it was admitted with, whatever
+        // the context's option does later
+        r.plan.nb = window_edge(ctx, get_sys(ctx, sys));
+        r.nwin = plan_windows(ctx, sys, r.plan.nb);
+        r.plan.measured = false;
+        r.plan.sec_window = plain_window_estimate(r.No, r.Nv, r.plan.nb,
+                                                  srv->precision, grid);
+      } else {
+        r.plan = plan_for(ctx, get_sys(ctx, sys), grid, srv->precision);
+        r.wins = windows_of(r.Nv, r.plan.nb);
+        r.nwin = r.wins.size();
+      }
+    } catch (const std::exception &x) {
+      // a request whose plan cannot be made fails alone and gives its
+      // memory back, rather than staying resident with no windows queued
+      drop_system(srv, r);
+      fail_request(srv, r, x.what());
+      continue;
+    } catch (const Fail &x) {
+      drop_system(srv, r);
+      fail_request(srv, r, x.msg);
+      continue;
     }
+    r.state = ACTIVE;
     r.cursor = 0;
     if (r.nwin == 0)
       retire(srv, r);
